@@ -1,0 +1,154 @@
+/*
+ * numax_hip.h -- C ABI of libnumax_hip.so, the MI355X (gfx950) kernels behind
+ * numax.linalg's dense factor-and-solve path.
+ *
+ * Conventions (all entry points):
+ *   - matrices are row-major device pointers with a leading dimension
+ *     (`lda` >= number of columns), as numax's `row_major` layouts;
+ *   - pivots are int32, 0-based, LAPACK `ipiv` semantics: row j was
+ *     interchanged with row ipiv[j], in order j = 0..n-1;
+ *   - `info` is a device int32 written LAPACK-style (1-based index of the
+ *     first failure, 0 if none), never raised inside a kernel;
+ *   - every call is asynchronous on the context's stream;
+ *   - return value: 0 on success, -i if argument i is invalid, a positive
+ *     NMX_ERR_* code otherwise.  No exception crosses the ABI.
+ *
+ * The reference (numax 0.3.0, Mojo) has no FFI; each declaration names the
+ * Mojo entry point whose device work it performs.
+ */
+#ifndef NUMAX_HIP_H
+#define NUMAX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NMX_OK 0
+#define NMX_ERR_HIP 1         /* a HIP runtime call failed */
+#define NMX_ERR_ALLOC 2       /* device allocation failed */
+#define NMX_ERR_UNSUPPORTED 3 /* size or option outside what the kernels handle */
+#define NMX_ERR_SYNC_TIMEOUT 4 /* a cross-workgroup wait hit its spin bound */
+#define NMX_ERR_NOT_CONVERGED 5 /* refinement did not reach its target */
+
+typedef struct nmx_ctx nmx_ctx;
+
+/* ---- context -------------------------------------------------------------
+ * Owns the stream handle (borrowed when `stream` is non-null), a grow-only
+ * device workspace and the cross-workgroup mailbox.  One context per host
+ * thread; calls on one context are not thread-safe.
+ * Replaces numax's DeviceContext plumbing (numax/linalg/panel.mojo:10-24). */
+int nmx_ctx_create(int device, void* stream, nmx_ctx** out);
+int nmx_ctx_destroy(nmx_ctx* ctx);
+int nmx_ctx_set_stream(nmx_ctx* ctx, void* stream);
+int nmx_sync(nmx_ctx* ctx);
+/* Sticky device-side error word of the context (NMX_ERR_SYNC_TIMEOUT after a
+ * bounded wait expired); reading it synchronizes the stream and clears it. */
+int nmx_ctx_check(nmx_ctx* ctx);
+const char* nmx_status_string(int status);
+int nmx_version(void);
+
+/* Optional per-kernel-class event timing on the context's stream (no
+ * reference counterpart: it serves the roofline report).  Turning it on or
+ * off clears the totals; nmx_ctx_profile synchronizes and returns the totals
+ * of class `kind` (0 .. 9, names from nmx_kernel_name). */
+int nmx_ctx_set_profiling(nmx_ctx* ctx, int on);
+int nmx_ctx_profile(nmx_ctx* ctx, int kind, int64_t* launches, double* ms, double* flops, double* bytes);
+const char* nmx_kernel_name(int kind);
+
+/* ---- LU with partial pivoting --------------------------------------------
+ * lu_factor[gpu=True] (numax/linalg/lu.mojo:454-701, runtime form :1200-1312):
+ * in-place P A = L U, unit L below the diagonal, first-maximum pivots, full
+ * row interchanges, an exact-zero pivot divided as 1e-30 and recorded in
+ * `info` (not an error).  `block` is the panel width, `base` the recursive
+ * panel's leaf width (0 picks the defaults 256 / 32). */
+int nmx_getrf_f64(nmx_ctx* ctx, int n, double* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
+int nmx_getrf_f32(nmx_ctx* ctx, int n, float* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
+
+/* LU.solve / DynamicLU.solve (lu.mojo:181-354, :1058-1124): laswp, then the
+ * unit-lower and upper blocked substitutions, in place on the n x nrhs B. */
+int nmx_getrs_f64(nmx_ctx* ctx, int n, int nrhs, const double* LU, int lda, const int32_t* ipiv, double* B, int ldb);
+int nmx_getrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* LU, int lda, const int32_t* ipiv, float* B, int ldb);
+
+/* laswp / laswp_matrix (numax/linalg/panel.mojo:1824-1852, :2279-2313). */
+int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, const int32_t* ipiv);
+
+/* LU._diagonal (lu.mojo:355-379): gather diag(A) into an n-vector for
+ * det / slogdet. */
+int nmx_diag_f64(nmx_ctx* ctx, int n, const double* A, int lda, double* diag);
+int nmx_diag_f32(nmx_ctx* ctx, int n, const float* A, int lda, float* diag);
+
+/* solve_triangular / _trsm (numax/linalg/triangular.mojo:114-225, :228-425):
+ * op(T) X = B in place, T n x n; `upper` is the triangle the substitution
+ * walks, `unit` ignores T's diagonal, `trans` reads T transposed. */
+int nmx_trsm_f64(nmx_ctx* ctx, int upper, int unit, int trans, int n, int nrhs, const double* T, int ldt, double* B,
+                 int ldb);
+int nmx_trsm_f32(nmx_ctx* ctx, int upper, int unit, int trans, int n, int nrhs, const float* T, int ldt, float* B,
+                 int ldb);
+
+/* ---- Cholesky ------------------------------------------------------------
+ * cholesky[gpu=True] (numax/linalg/cholesky.mojo:59-396): lower L with
+ * L L^T = A reading only A's lower triangle; a non-positive pivot is floored
+ * to 1e-30 and its 1-based index recorded in `info` (the caller raises);
+ * the strict upper triangle of the result is zeroed (tril). */
+int nmx_potrf_f64(nmx_ctx* ctx, int n, double* A, int lda, int32_t* info, int block);
+int nmx_potrf_f32(nmx_ctx* ctx, int n, float* A, int lda, int32_t* info, int block);
+/* cholesky_solve (cholesky.mojo:471-580): L y = b, then L^T x = y. */
+int nmx_potrs_f64(nmx_ctx* ctx, int n, int nrhs, const double* L, int lda, double* B, int ldb);
+int nmx_potrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* L, int lda, float* B, int ldb);
+
+/* ---- GEMM update used by the drivers (exported for tests) ----------------
+ * C = beta C + alpha op(A) op(B), op(A) m x k, op(B) k x n.  The trailing
+ * update `A22 -= L21 U12` of lu.mojo:629-673 is alpha = -1, beta = 1. */
+int nmx_gemm_f64(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
+                 const double* B, int ldb, double beta, double* C, int ldc);
+int nmx_gemm_f32(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, float alpha, const float* A, int lda,
+                 const float* B, int ldb, float beta, float* C, int ldc);
+
+/* ---- mixed-precision LU with iterative refinement --------------------------
+ * No reference counterpart (numax has no mixed precision and no refinement);
+ * it serves `solve(a, b)` (numax/linalg/basic.mojo:55-146) when a caller asks
+ * for FP64 accuracy at low-precision factorization cost.  A' = W^T A V with
+ * random butterflies W, V; no-pivot blocked LU of A' in FP32 with the
+ * trailing updates on FP16 / BF16 MFMA (FP32 accumulate, power-of-two operand
+ * scaling); then r = b - A x in FP64, x += V U^-1 L^-1 W^T r until the
+ * normwise backward error ||r||_inf / (||A||_inf ||x||_inf) <= tol.
+ * A is read, never written; x receives the solution. */
+typedef struct nmx_ir_cfg {
+  int precision;     /* trailing update: 0 fp16, 1 bf16, 2 fp16 split (hi + 2^-11 lo, 3 products), 3 fp32 */
+  int block;         /* panel width, a multiple of 64 (0: 256) */
+  int max_iter;      /* refinement steps at most */
+  int solver;        /* 0 auto, 1 classical refinement, 2 GMRES-IR */
+  int gmres_restart; /* GMRES restart length */
+  double tol;        /* target backward error */
+  uint64_t seed;     /* butterfly seed */
+} nmx_ir_cfg;
+
+typedef struct nmx_ir_report {
+  int iterations;       /* refinement steps taken */
+  int gmres_iterations; /* inner GMRES iterations (GMRES-IR) */
+  int converged;        /* backward_error <= tol */
+  int info;             /* first exactly-zero pivot (1-based) of the low-precision factor, 0 if none */
+  double backward_error;
+  double factor_ms;
+  double refine_ms;
+  double history[32];   /* backward error after each step */
+} nmx_ir_report;
+
+void nmx_ir_cfg_default(nmx_ir_cfg* cfg);
+int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x, const nmx_ir_cfg* cfg,
+                  nmx_ir_report* report);
+
+/* ---- synthetic inputs ------------------------------------------------------
+ * Philox4x32-10 stream (the same words as numax_amd.random): kind 0 fills
+ * lo + (hi - lo) U[0,1), kind 1 fills N(0,1) by Box--Muller.  Element e of
+ * `out` is stream element offset + e. */
+int nmx_fill_random_f64(nmx_ctx* ctx, int kind, uint64_t seed, int64_t offset, int64_t count, double lo, double hi,
+                        double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUMAX_HIP_H */

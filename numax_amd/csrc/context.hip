@@ -1,0 +1,267 @@
+// Context, workspace, mailbox, status strings and the Philox input generator.
+#include <math.h>
+#include <stdlib.h>
+
+#include "nmx_internal.h"
+
+NMX_API int nmx_version(void) { return 1; }
+
+NMX_API const char* nmx_status_string(int status) {
+  if (status < 0) return "invalid argument";
+  switch (status) {
+    case NMX_OK: return "ok";
+    case NMX_ERR_HIP: return "HIP runtime error";
+    case NMX_ERR_ALLOC: return "device allocation failed";
+    case NMX_ERR_UNSUPPORTED: return "unsupported size or option";
+    case NMX_ERR_SYNC_TIMEOUT: return "cross-workgroup wait timed out";
+    case NMX_ERR_NOT_CONVERGED: return "iterative refinement did not converge";
+    default: return "unknown status";
+  }
+}
+
+__global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
+  if (!out) return -3;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return NMX_ERR_HIP;
+  if (device < 0 || device >= count) return -1;
+  nmx_ctx* c = new nmx_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return NMX_ERR_HIP; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NMX_ERR_HIP; }
+  c->num_cus = prop.multiProcessorCount;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return NMX_ERR_HIP; }
+    c->own_stream = true;
+  }
+  if (hipMalloc(&c->err, 256) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
+  if (hipMalloc(&c->mailbox, kMailboxBytes) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
+  c->mailbox_bytes = kMailboxBytes;
+  size_t words = kMailboxBytes / 4;
+  fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
+  fill_u32<<<1, 64, 0, c->stream>>>((uint32_t*)c->err, 64, 0u);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+    nmx_ctx_destroy(c);
+    return NMX_ERR_HIP;
+  }
+  *out = c;
+  return NMX_OK;
+}
+
+NMX_API int nmx_ctx_destroy(nmx_ctx* c) {
+  if (!c) return -1;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto e : c->pool) hipEventDestroy(e);
+  if (c->ws) hipFree(c->ws);
+  if (c->mailbox) hipFree(c->mailbox);
+  if (c->err) hipFree(c->err);
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return NMX_OK;
+}
+
+NMX_API int nmx_ctx_set_stream(nmx_ctx* c, void* stream) {
+  if (!c) return -1;
+  if (c->own_stream && c->stream) {
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+    c->own_stream = false;
+  }
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    NMX_HIP(c, hipStreamCreate(&c->stream));
+    c->own_stream = true;
+  }
+  return NMX_OK;
+}
+
+NMX_API int nmx_sync(nmx_ctx* c) {
+  if (!c) return -1;
+  NMX_HIP(c, hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+NMX_API int nmx_ctx_check(nmx_ctx* c) {
+  if (!c) return -1;
+  int h = 0;
+  NMX_HIP(c, hipMemcpyAsync(&h, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  NMX_HIP(c, hipStreamSynchronize(c->stream));
+  if (h != 0) {
+    fill_u32<<<1, 64, 0, c->stream>>>((uint32_t*)c->err, 1, 0u);
+    NMX_HIP(c, hipGetLastError());
+    NMX_HIP(c, hipStreamSynchronize(c->stream));
+    return h;
+  }
+  return NMX_OK;
+}
+
+int nmx_workspace(nmx_ctx* c, size_t bytes, void** out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes > c->ws_bytes) {
+    NMX_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->ws) hipFree(c->ws);
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    if (hipMalloc(&c->ws, bytes) != hipSuccess) return NMX_ERR_ALLOC;
+    c->ws_bytes = bytes;
+  }
+  *out = c->ws;
+  return NMX_OK;
+}
+
+int nmx_mailbox(nmx_ctx* c, void** out) {
+  *out = c->mailbox;
+  return NMX_OK;
+}
+
+int nmx_epoch_reserve(nmx_ctx* c, uint32_t columns, uint32_t* base) {
+  if ((uint64_t)c->epoch + columns + 4 >= 0x7fffffffu) {
+    size_t words = kMailboxFlagsBytes / 4;
+    fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
+    NMX_HIP(c, hipGetLastError());
+    c->epoch = 0;
+  }
+  *base = c->epoch;
+  c->epoch += columns;
+  return NMX_OK;
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) --------------------------------
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; r++) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+  }
+}
+
+__device__ __forceinline__ double philox_u01(uint64_t seed, uint64_t word) {
+  uint64_t off = word >> 1;
+  uint32_t c[4] = {(uint32_t)off, (uint32_t)(off >> 32), 0u, 0u};
+  philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  int pair = (int)(word & 1) * 2;
+  uint64_t bits = ((uint64_t)c[pair] << 32) | c[pair + 1];
+  return (double)(bits >> 11) * 1.1102230246251565e-16;
+}
+
+__global__ void fill_random_kernel(int kind, uint64_t seed, int64_t offset, int64_t count, double lo, double hi,
+                                   double* out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  uint64_t idx = (uint64_t)(offset + e);
+  if (kind == 0) {
+    // unfused, so the value matches the host stream bit for bit
+    out[e] = __dadd_rn(lo, __dmul_rn(__dadd_rn(hi, -lo), philox_u01(seed, idx)));
+  } else {
+    // Both uniforms of normal i sit in Philox block i.
+    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u};
+    philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t b1 = ((uint64_t)c[0] << 32) | c[1];
+    uint64_t b2 = ((uint64_t)c[2] << 32) | c[3];
+    double u1 = (double)(b1 >> 11) * 1.1102230246251565e-16;
+    double u2 = (double)(b2 >> 11) * 1.1102230246251565e-16;
+    out[e] = sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
+  }
+}
+
+NMX_API int nmx_fill_random_f64(nmx_ctx* c, int kind, uint64_t seed, int64_t offset, int64_t count, double lo,
+                                double hi, double* out) {
+  if (!c) return -1;
+  if (kind != 0 && kind != 1) return -2;
+  if (count < 0) return -5;
+  if (count == 0) return NMX_OK;
+  if (!out) return -8;
+  int64_t blocks = (count + 255) / 256;
+  fill_random_kernel<<<(unsigned)blocks, 256, 0, c->stream>>>(kind, seed, offset, count, lo, hi, out);
+  NMX_HIP(c, hipGetLastError());
+  return NMX_OK;
+}
+
+// ---- event profiler ----------------------------------------------------------
+static hipEvent_t pool_get(nmx_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+ProfScope::ProfScope(nmx_ctx* c, int kind, double flops, double bytes) : ctx(c), on(c && c->profiling) {
+  if (!on) return;
+  rec.kind = kind;
+  rec.flops = flops;
+  rec.bytes = bytes;
+  rec.a = pool_get(c);
+  rec.b = pool_get(c);
+  hipEventRecord(rec.a, c->stream);
+}
+
+ProfScope::~ProfScope() {
+  if (!on) return;
+  hipEventRecord(rec.b, ctx->stream);
+  ctx->recs.push_back(rec);
+}
+
+static int prof_collect(nmx_ctx* c) {
+  if (c->recs.empty()) return NMX_OK;
+  NMX_HIP(c, hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->prof_launches[r.kind] += 1;
+      c->prof_ms[r.kind] += ms;
+      c->prof_flops[r.kind] += r.flops;
+      c->prof_bytes[r.kind] += r.bytes;
+    }
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  return NMX_OK;
+}
+
+NMX_API int nmx_ctx_set_profiling(nmx_ctx* c, int on) {
+  if (!c) return -1;
+  NMX_TRY(prof_collect(c));
+  c->profiling = on != 0;
+  for (int k = 0; k < K_COUNT; k++) {
+    c->prof_launches[k] = 0;
+    c->prof_ms[k] = c->prof_flops[k] = c->prof_bytes[k] = 0.0;
+  }
+  return NMX_OK;
+}
+
+NMX_API int nmx_ctx_profile(nmx_ctx* c, int kind, int64_t* launches, double* ms, double* flops, double* bytes) {
+  if (!c) return -1;
+  if (kind < 0 || kind >= K_COUNT) return -2;
+  NMX_TRY(prof_collect(c));
+  if (launches) *launches = c->prof_launches[kind];
+  if (ms) *ms = c->prof_ms[kind];
+  if (flops) *flops = c->prof_flops[kind];
+  if (bytes) *bytes = c->prof_bytes[kind];
+  return NMX_OK;
+}
+
+NMX_API const char* nmx_kernel_name(int kind) {
+  static const char* names[K_COUNT] = {"gemm",       "getf2_leaf", "laswp",    "trsm_base", "potrf_diag",
+                                       "mp_gemm",    "mp_panel",   "residual", "mp_trsv",   "other"};
+  return (kind >= 0 && kind < K_COUNT) ? names[kind] : "unknown";
+}

@@ -1,0 +1,262 @@
+// MFMA GEMM update C = beta C + alpha op(A) op(B) for the blocked drivers.
+//
+// This is the trailing update numax hands to MAX's linalg.matmul with a
+// subtracting epilogue (numax/linalg/lu.mojo:629-673, panel.mojo:557-582,
+// cholesky.mojo:126-225, triangular.mojo:193-218, qr.mojo:262-327).  Unlike
+// MAX's matmul it reads strided operands in place, so the drivers never pack
+// L21 / U12, and it writes the epilogue straight into the strided matrix.
+//
+// Tiling for CDNA4: a 256-thread workgroup owns a 128x128 tile of C, four
+// 64-wide wavefronts in a 2x2 arrangement each owning 64x64 = 4x4 MFMA
+// 16x16 tiles.  K advances 16 at a time through double-buffered LDS with the
+// next K slab prefetched into registers while the current one is consumed.
+// FP64 runs v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32.
+//
+// Tile order is XCD-aware: workgroup b runs on XCD b % 8, so each XCD is
+// given a contiguous run of tiles (grouped 8 tile-rows at a time) and the
+// A row-panels and B column-panels it reads stay in that XCD's L2.
+#include "nmx_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // Output item i of lane l sits at row l/16 + 4 i (measured on gfx950).
+  static __device__ __forceinline__ int row(int lane, int i) { return (lane >> 4) + 4 * i; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+
+template <typename T, int TA, int TB>
+__global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
+                                                   const T* __restrict__ B, int ldb, T beta, T* __restrict__ C,
+                                                   int ldc, int tilesM, int tilesN, int lower_only) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[2][BK][BM + PAD];
+  __shared__ T Bs[2][BK][BN + PAD];
+
+  // ---- tile assignment: contiguous chunk per XCD, grouped rows inside ----
+  const int total = lower_only ? tilesM * (tilesM + 1) / 2 : tilesM * tilesN;
+  const int b = blockIdx.x;
+  const int per = (total + kXcds - 1) / kXcds;
+  const int lin = (b % kXcds) * per + b / kXcds;
+  if (lin >= total) return;
+  int tm, tn;
+  if (lower_only) {
+    // lin -> (tm, tn) with tn <= tm, row-major over the lower block triangle
+    int r = (int)((sqrtf(8.0f * (float)lin + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= lin) r++;
+    while (r * (r + 1) / 2 > lin) r--;
+    tm = r;
+    tn = lin - r * (r + 1) / 2;
+  } else {
+    const int GM = 8;
+    int group = lin / (GM * tilesN);
+    int first = group * GM;
+    int gm = min(tilesM - first, GM);
+    int in = lin % (GM * tilesN);
+    tm = first + in % gm;
+    tn = in / gm;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  T ra[8], rb[8];
+  auto load_tiles = [&](int k0) {
+    // op(A) tile BM x BK
+    if (!TA) {
+      int r = t >> 1, kh = (t & 1) * 8, gm = m0 + r;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        int gk = k0 + kh + i;
+        ra[i] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : T(0);
+      }
+    } else {
+      int kk = t >> 4, mc = (t & 15) * 8, gk = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        int gm = m0 + mc + i;
+        ra[i] = (gm < M && gk < K) ? A[(int64_t)gk * lda + gm] : T(0);
+      }
+    }
+    // op(B) tile BK x BN
+    if (!TB) {
+      int kk = t >> 4, nc = (t & 15) * 8, gk = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        int gn = n0 + nc + i;
+        rb[i] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : T(0);
+      }
+    } else {
+      int r = t >> 1, kh = (t & 1) * 8, gn = n0 + r;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        int gk = k0 + kh + i;
+        rb[i] = (gn < N && gk < K) ? B[(int64_t)gn * ldb + gk] : T(0);
+      }
+    }
+  };
+  auto store_tiles = [&](int buf) {
+    if (!TA) {
+      int r = t >> 1, kh = (t & 1) * 8;
+#pragma unroll
+      for (int i = 0; i < 8; i++) As[buf][kh + i][r] = ra[i];
+    } else {
+      int kk = t >> 4, mc = (t & 15) * 8;
+#pragma unroll
+      for (int i = 0; i < 8; i++) As[buf][kk][mc + i] = ra[i];
+    }
+    if (!TB) {
+      int kk = t >> 4, nc = (t & 15) * 8;
+#pragma unroll
+      for (int i = 0; i < 8; i++) Bs[buf][kk][nc + i] = rb[i];
+    } else {
+      int r = t >> 1, kh = (t & 1) * 8;
+#pragma unroll
+      for (int i = 0; i < 8; i++) Bs[buf][kh + i][r] = rb[i];
+    }
+  };
+
+  const int ksteps = (K + BK - 1) / BK;
+  if (ksteps > 0) {
+    load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+  }
+  for (int s = 0; s < ksteps; s++) {
+    const int buf = s & 1;
+    if (s + 1 < ksteps) load_tiles((s + 1) * BK);
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; kq++) {
+      const int kr = kq * 4 + (lane >> 4);
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) af[i] = As[buf][kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bf[j] = Bs[buf][kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+    }
+    if (s + 1 < ksteps) {
+      store_tiles(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- epilogue straight into the strided C ----
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e);
+        if (row < M && col < N) {
+          T* cp = C + (int64_t)row * ldc + col;
+          T v = alpha * acc[i][j][e];
+          if (beta != T(0)) v += beta * *cp;
+          *cp = v;
+        }
+      }
+    }
+}
+
+template <typename T>
+__global__ void scale_kernel(int M, int N, T beta, T* C, int ldc) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)M * N) return;
+  int r = (int)(idx / N), c = (int)(idx % N);
+  T* p = C + (int64_t)r * ldc + c;
+  *p = beta == T(0) ? T(0) : beta * *p;
+}
+
+}  // namespace
+
+template <typename T>
+int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alpha, const T* A, int lda, const T* B,
+                int ldb, T beta, T* C, int ldc, int lower_only) {
+  if (m <= 0 || n <= 0) return NMX_OK;
+  if (k <= 0 || alpha == T(0)) {
+    if (beta == T(1)) return NMX_OK;
+    int64_t total = (int64_t)m * n;
+    scale_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(m, n, beta, C, ldc);
+    NMX_HIP(ctx, hipGetLastError());
+    return NMX_OK;
+  }
+  if (lower_only && m != n) return NMX_ERR_UNSUPPORTED;
+  int tilesM = ceil_div(m, BM), tilesN = ceil_div(n, BN);
+  int total = lower_only ? tilesM * (tilesM + 1) / 2 : tilesM * tilesN;
+  int grid = ceil_div(total, kXcds) * kXcds;
+  dim3 g(grid), blk(256);
+  const double flops = 2.0 * m * (double)n * k * (lower_only ? 0.5 : 1.0);
+  ProfScope ps(ctx, K_GEMM, flops, (double)sizeof(T) * ((double)m * k + (double)k * n + 2.0 * m * n));
+  if (!transa && !transb)
+    gemm_kernel<T, 0, 0><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+  else if (!transa && transb)
+    gemm_kernel<T, 0, 1><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+  else if (transa && !transb)
+    gemm_kernel<T, 1, 0><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+  else
+    gemm_kernel<T, 1, 1><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+template int gemm_launch<double>(nmx_ctx*, int, int, int, int, int, double, const double*, int, const double*, int,
+                                 double, double*, int, int);
+template int gemm_launch<float>(nmx_ctx*, int, int, int, int, int, float, const float*, int, const float*, int, float,
+                                float*, int, int);
+
+template <typename T>
+static int gemm_api(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alpha, const T* A, int lda,
+                    const T* B, int ldb, T beta, T* C, int ldc) {
+  if (!ctx) return -1;
+  if (m < 0) return -4;
+  if (n < 0) return -5;
+  if (k < 0) return -6;
+  if (lda < (transa ? m : k) || lda < 1) return -9;
+  if (ldb < (transb ? k : n) || ldb < 1) return -11;
+  if (ldc < n || ldc < 1) return -14;
+  return gemm_launch<T>(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+NMX_API int nmx_gemm_f64(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, double alpha, const double* A,
+                         int lda, const double* B, int ldb, double beta, double* C, int ldc) {
+  return gemm_api<double>(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+NMX_API int nmx_gemm_f32(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, float alpha, const float* A,
+                         int lda, const float* B, int ldb, float beta, float* C, int ldc) {
+  return gemm_api<float>(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}

@@ -1,0 +1,909 @@
+// Mixed-precision LU with iterative refinement (no reference counterpart;
+// the reference has no mixed precision and no refinement).
+//
+// Factor:  A' = W^T A V with W, V depth-2 random butterflies (Parker's RBT;
+//          Baboulin, Dongarra et al.), stored FP32, then blocked LU without
+//          pivoting.  The diagonal block and both panel solves run in FP32;
+//          the trailing update A22 -= L21 U12 -- the cubic term -- runs on
+//          FP16 or BF16 MFMA with FP32 accumulation, the operands scaled by
+//          powers of two so their largest entry sits at 2^14 (Higham, Pranesh
+//          & Zounon 2019).  The "split" update keeps a second FP16 term per
+//          operand (x ~ hi + 2^-11 lo) and adds the two cross products, which
+//          recovers ~FP32 accuracy for 3x the MFMA work -- free while the
+//          update is bound by its FP32 C traffic.
+// Refine:  r = b - A x in FP64 on the original matrix, d = V U^-1 L^-1 W^T r
+//          in FP32, x += d in FP64, until the normwise backward error meets
+//          the target.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "nmx_internal.h"
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+constexpr float kRsqrt2 = 0.70710678118654752440f;
+constexpr float kPivotFloorF = 1e-30f;
+
+// ---------------------------------------------------------------------------
+// Random butterflies.  A depth-2 butterfly closes over index groups
+// {i, i+h, i+2h, i+3h} (h = n/4), so W^T A V is computed 4x4 block by 4x4
+// block: T_V on each row, then T_W on each column.  Diagonals per transform
+// (2n floats): r0, r1 (level 0, n/2 each), s0, s1, t0, t1 (level 1, n/4 each).
+// ---------------------------------------------------------------------------
+struct Bfly {
+  const float* r0;
+  const float* r1;
+  const float* s0;
+  const float* s1;
+  const float* t0;
+  const float* t1;
+};
+
+__device__ __forceinline__ Bfly bfly_of(const float* d, int n) {
+  const int half = n / 2, q = n / 4;
+  return Bfly{d, d + half, d + 2 * half, d + 2 * half + q, d + 2 * half + 2 * q, d + 2 * half + 3 * q};
+}
+
+// y <- B^T y on group i (the transpose transform).
+template <typename T>
+__device__ __forceinline__ void bfly_t(T y[4], const Bfly& f, int i, int h) {
+  T z0 = f.s0[i] * (y[0] + y[1]) * kRsqrt2;
+  T z1 = f.s1[i] * (y[0] - y[1]) * kRsqrt2;
+  T z2 = f.t0[i] * (y[2] + y[3]) * kRsqrt2;
+  T z3 = f.t1[i] * (y[2] - y[3]) * kRsqrt2;
+  y[0] = f.r0[i] * (z0 + z2) * kRsqrt2;
+  y[2] = f.r1[i] * (z0 - z2) * kRsqrt2;
+  y[1] = f.r0[i + h] * (z1 + z3) * kRsqrt2;
+  y[3] = f.r1[i + h] * (z1 - z3) * kRsqrt2;
+}
+
+// y <- B y on group i (level 0 first, then level 1).
+template <typename T>
+__device__ __forceinline__ void bfly_n(T y[4], const Bfly& f, int i, int h) {
+  T z0 = (f.r0[i] * y[0] + f.r1[i] * y[2]) * kRsqrt2;
+  T z2 = (f.r0[i] * y[0] - f.r1[i] * y[2]) * kRsqrt2;
+  T z1 = (f.r0[i + h] * y[1] + f.r1[i + h] * y[3]) * kRsqrt2;
+  T z3 = (f.r0[i + h] * y[1] - f.r1[i + h] * y[3]) * kRsqrt2;
+  y[0] = (f.s0[i] * z0 + f.s1[i] * z1) * kRsqrt2;
+  y[1] = (f.s0[i] * z0 - f.s1[i] * z1) * kRsqrt2;
+  y[2] = (f.t0[i] * z2 + f.t1[i] * z3) * kRsqrt2;
+  y[3] = (f.t0[i] * z2 - f.t1[i] * z3) * kRsqrt2;
+}
+
+// A' (FP32) = W^T A V for the leading n4 = 4*floor(n/4) block; the tail rows
+// and columns pass through (the transform is diag(W, I) / diag(V, I)).
+__global__ void rbt_matrix_kernel(int n, const double* __restrict__ A, int lda, float* __restrict__ Ap, int ldp,
+                                  const float* __restrict__ dw, const float* __restrict__ dv) {
+  const int n4 = n & ~3, h = n4 / 4;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // groups: h x h interior blocks, then the pass-through tail
+  if (idx < (int64_t)h * h) {
+    const int gi = (int)(idx / h), gj = (int)(idx % h);
+    const Bfly fw = bfly_of(dw, n4), fv = bfly_of(dv, n4);
+    float x[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      float row[4];
+#pragma unroll
+      for (int b = 0; b < 4; b++) row[b] = (float)A[(int64_t)(gi + a * h) * lda + gj + b * h];
+      bfly_t(row, fv, gj, h);  // (A V) row = V^T applied to the row as a column
+#pragma unroll
+      for (int b = 0; b < 4; b++) x[a][b] = row[b];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      float col[4] = {x[0][b], x[1][b], x[2][b], x[3][b]};
+      bfly_t(col, fw, gi, h);
+#pragma unroll
+      for (int a = 0; a < 4; a++) Ap[(int64_t)(gi + a * h) * ldp + gj + b * h] = col[a];
+    }
+    return;
+  }
+  // tail: rows >= n4 (all columns) and columns >= n4 of rows < n4
+  int64_t t = idx - (int64_t)h * h;
+  const int tail = n - n4;
+  if (tail == 0) return;
+  const int64_t rows_tail = (int64_t)tail * n;
+  if (t < rows_tail) {
+    int r = n4 + (int)(t / n), c = (int)(t % n);
+    // row r untransformed on the left; columns transformed by V on groups
+    if (c < n4) {
+      const Bfly fv = bfly_of(dv, n4);
+      int gj = c % h, b = c / h;
+      float row[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) row[q] = (float)A[(int64_t)r * lda + gj + q * h];
+      bfly_t(row, fv, gj, h);
+      Ap[(int64_t)r * ldp + c] = row[b];
+    } else {
+      Ap[(int64_t)r * ldp + c] = (float)A[(int64_t)r * lda + c];
+    }
+    return;
+  }
+  t -= rows_tail;
+  if (t < (int64_t)n4 * tail) {
+    int r = (int)(t / tail), c = n4 + (int)(t % tail);
+    const Bfly fw = bfly_of(dw, n4);
+    int gi = r % h, a = r / h;
+    float col[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) col[q] = (float)A[(int64_t)(gi + q * h) * lda + c];
+    bfly_t(col, fw, gi, h);
+    Ap[(int64_t)r * ldp + c] = col[a];
+  }
+}
+
+// v (FP64 in, FP32 out) <- W^T v, per column of an n x nrhs block.
+__global__ void rbt_vec_t_kernel(int n, int nrhs, const double* __restrict__ v, int ldv, float* __restrict__ out,
+                                 int ldo, const float* __restrict__ dw) {
+  const int n4 = n & ~3, h = n4 / 4;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t groups = (int64_t)h * nrhs;
+  if (idx < groups) {
+    const int gi = (int)(idx / nrhs), c = (int)(idx % nrhs);
+    const Bfly fw = bfly_of(dw, n4);
+    float y[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) y[a] = (float)v[(int64_t)(gi + a * h) * ldv + c];
+    bfly_t(y, fw, gi, h);
+#pragma unroll
+    for (int a = 0; a < 4; a++) out[(int64_t)(gi + a * h) * ldo + c] = y[a];
+  } else {
+    int64_t t = idx - groups;
+    if (t < (int64_t)(n - n4) * nrhs) {
+      int r = n4 + (int)(t / nrhs), c = (int)(t % nrhs);
+      out[(int64_t)r * ldo + c] = (float)v[(int64_t)r * ldv + c];
+    }
+  }
+}
+
+// x (FP64) += V y (y FP32), per column.
+__global__ void rbt_vec_n_add_kernel(int n, int nrhs, const float* __restrict__ y, int ldy, double* __restrict__ x,
+                                     int ldx, const float* __restrict__ dv, int accumulate) {
+  const int n4 = n & ~3, h = n4 / 4;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t groups = (int64_t)h * nrhs;
+  if (idx < groups) {
+    const int gi = (int)(idx / nrhs), c = (int)(idx % nrhs);
+    const Bfly fv = bfly_of(dv, n4);
+    double z[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) z[a] = (double)y[(int64_t)(gi + a * h) * ldy + c];
+    bfly_n(z, fv, gi, h);
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      double* p = x + (int64_t)(gi + a * h) * ldx + c;
+      *p = accumulate ? *p + z[a] : z[a];
+    }
+  } else {
+    int64_t t = idx - groups;
+    if (t < (int64_t)(n - n4) * nrhs) {
+      int r = n4 + (int)(t / nrhs), c = (int)(t % nrhs);
+      double* p = x + (int64_t)r * ldx + c;
+      double z = (double)y[(int64_t)r * ldy + c];
+      *p = accumulate ? *p + z : z;
+    }
+  }
+}
+
+// Butterfly diagonals exp(rho/10), rho ~ U[-1/2, 1/2) from Philox.
+__device__ __forceinline__ uint32_t philox_word(uint64_t seed, uint64_t idx) {
+  uint32_t c[4] = {(uint32_t)(idx >> 2), (uint32_t)(idx >> 34), 0x52425400u, 0u};
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int r = 0; r < 10; r++) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+  }
+  return c[idx & 3];
+}
+
+__global__ void rbt_diag_kernel(int count, uint64_t seed, float* __restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float u = (float)(philox_word(seed, (uint64_t)i) >> 8) * 5.9604644775390625e-08f - 0.5f;
+  d[i] = expf(u * 0.1f);
+}
+
+// ---------------------------------------------------------------------------
+// No-pivot diagonal leaf (nb <= 64) in LDS: right-looking LU, an exact-zero
+// pivot floored to 1e-30 with its first occurrence in info.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) getf2_nopiv_kernel(float* __restrict__ A, int lda, int k0, int nb,
+                                                          int32_t* __restrict__ info) {
+  __shared__ float S[64][65];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < nb * nb; idx += 256) S[idx / nb][idx % nb] = A[(int64_t)(k0 + idx / nb) * lda + k0 + idx % nb];
+  __syncthreads();
+  for (int j = 0; j < nb; j++) {
+    float p = S[j][j];
+    if (p == 0.0f) {
+      if (t == 0 && *info == 0) *info = k0 + j + 1;
+      p = kPivotFloorF;
+    }
+    for (int i = j + 1 + t; i < nb; i += 256) S[i][j] = S[i][j] / p;
+    __syncthreads();
+    for (int idx = t; idx < (nb - j - 1) * (nb - j - 1); idx += 256) {
+      int i = j + 1 + idx / (nb - j - 1), c = j + 1 + idx % (nb - j - 1);
+      S[i][c] = S[i][c] - S[i][j] * S[j][c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < nb * nb; idx += 256) A[(int64_t)(k0 + idx / nb) * lda + k0 + idx % nb] = S[idx / nb][idx % nb];
+}
+
+// ---------------------------------------------------------------------------
+// Trailing-update operands: |max| -> power-of-two scale, then FP16 / BF16
+// (hi [, lo]) copies.  L21 stays m x k (row-major); U12 (k x m) is stored
+// transposed (m x k) so both MFMA operands read 8 consecutive K values.
+// ---------------------------------------------------------------------------
+__global__ void absmax_kernel(int rows, int cols, const float* __restrict__ X, int ldx, unsigned* __restrict__ out) {
+  float m = 0.f;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    float v = fabsf(X[(int64_t)(idx / cols) * ldx + idx % cols]);
+    if (v > m) m = v;  // NaN never raises the scale
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__device__ __forceinline__ float pow2_scale(unsigned maxbits) {
+  float m = __uint_as_float(maxbits);
+  if (!(m > 0.f) || !isfinite(m)) return 1.0f;
+  int e;
+  frexpf(m, &e);          // m = f * 2^e, f in [0.5, 1)
+  return ldexpf(1.0f, 14 - e);  // max * scale in [2^13, 2^14)
+}
+
+template <int BF>
+struct Lo;
+template <>
+struct Lo<0> {
+  typedef _Float16 t;
+  static __device__ __forceinline__ t cvt(float x) { return (_Float16)x; }
+  static __device__ __forceinline__ float back(t x) { return (float)x; }
+};
+template <>
+struct Lo<1> {
+  typedef __bf16 t;
+  static __device__ __forceinline__ t cvt(float x) { return (__bf16)x; }
+  static __device__ __forceinline__ float back(t x) { return (float)x; }
+};
+
+// dst[r][c] = scaled src[r][c] (transpose = 0) or scaled src[c][r] (transpose = 1);
+// dst is rows x cols row-major with ld = cols.
+template <int BF, int SPLIT>
+__global__ void convert_kernel(int rows, int cols, const float* __restrict__ src, int lds, int transpose,
+                               const unsigned* __restrict__ maxbits, typename Lo<BF>::t* __restrict__ hi,
+                               typename Lo<BF>::t* __restrict__ lo) {
+  const float s = pow2_scale(*maxbits);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(idx / cols), c = (int)(idx % cols);
+    float v = (transpose ? src[(int64_t)c * lds + r] : src[(int64_t)r * lds + c]) * s;
+    typename Lo<BF>::t h = Lo<BF>::cvt(v);
+    hi[idx] = h;
+    if (SPLIT) lo[idx] = Lo<BF>::cvt((v - Lo<BF>::back(h)) * 2048.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C (FP32, in place) -= (A . B^T) / (sA sB) with A m x k, B n x k in FP16 /
+// BF16, FP32 accumulation on v_mfma_f32_32x32x16_{f16,bf16}.  128x128 C tile
+// per 256-thread workgroup, 64x64 per wavefront (2x2 MFMA tiles); the C tile
+// is prefetched into registers at the start so its HBM read overlaps the
+// MFMA work.  SPLIT adds 2^-11 (A_hi B_lo^T + A_lo B_hi^T).
+// ---------------------------------------------------------------------------
+template <int BF>
+struct Mma;
+template <>
+struct Mma<0> {
+  static __device__ __forceinline__ f16v op(h8 a, h8 b, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mma<1> {
+  static __device__ __forceinline__ f16v op(b8 a, b8 b, f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int MT = 128, KT = 32, KP = 8;  // C tile, K slab, LDS pad (halves)
+
+template <int BF, int SPLIT>
+__global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const typename Lo<BF>::t* __restrict__ Ah,
+                                                      const typename Lo<BF>::t* __restrict__ Al,
+                                                      const typename Lo<BF>::t* __restrict__ Bh,
+                                                      const typename Lo<BF>::t* __restrict__ Bl,
+                                                      const unsigned* __restrict__ maxA,
+                                                      const unsigned* __restrict__ maxB, float* __restrict__ C, int ldc,
+                                                      int tilesM, int tilesN) {
+  typedef typename Lo<BF>::t lt;
+  typedef lt v8 __attribute__((ext_vector_type(8)));
+  constexpr int NOP = SPLIT ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) lt As[NOP][MT][KT + KP];
+  __shared__ __attribute__((aligned(16))) lt Bs[NOP][MT][KT + KP];
+
+  const int total = tilesM * tilesN;
+  const int b = blockIdx.x;
+  const int per = (total + kXcds - 1) / kXcds;
+  const int lin = (b % kXcds) * per + b / kXcds;
+  if (lin >= total) return;
+  const int GM = 8;
+  const int group = lin / (GM * tilesN);
+  const int first = group * GM;
+  const int gm = min(tilesM - first, GM);
+  const int in = lin % (GM * tilesN);
+  const int m0 = (first + in % gm) * MT, n0 = (in / gm) * MT;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // prefetch this lane's C values (the MFMA output layout)
+  float cpre[2][2][16];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int row = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+        cpre[i][j][e] = (row < M && col < N) ? C[(int64_t)row * ldc + col] : 0.f;
+      }
+
+  f16v acc[2][2], accx[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = f16v{};
+      accx[i][j] = f16v{};
+    }
+
+  // global -> LDS: each operand slab is 128 rows x 32 halves = 512 chunks of 8
+  for (int k0 = 0; k0 < K; k0 += KT) {
+#pragma unroll
+    for (int op = 0; op < NOP; op++) {
+      const lt* srcA = op == 0 ? Ah : Al;
+      const lt* srcB = op == 0 ? Bh : Bl;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const int chunk = t + c * 256;
+        const int r = chunk >> 2, kk = (chunk & 3) * 8;
+        v8 va, vb;
+        const int gr = m0 + r, gc = n0 + r, gk = k0 + kk;
+        if (gr < M && gk + 8 <= K) {
+          va = *(const v8*)(srcA + (int64_t)gr * K + gk);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; q++) va[q] = (gr < M && gk + q < K) ? srcA[(int64_t)gr * K + gk + q] : (lt)0.0f;
+        }
+        if (gc < N && gk + 8 <= K) {
+          vb = *(const v8*)(srcB + (int64_t)gc * K + gk);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; q++) vb[q] = (gc < N && gk + q < K) ? srcB[(int64_t)gc * K + gk + q] : (lt)0.0f;
+        }
+        *(v8*)&As[op][r][kk] = va;
+        *(v8*)&Bs[op][r][kk] = vb;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KT; ks += 16) {
+      const int ko = ks + 8 * (lane >> 5);
+      v8 a0[2], b0[2], a1[2], b1[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        a0[i] = *(const v8*)&As[0][wm * 64 + i * 32 + (lane & 31)][ko];
+        b0[i] = *(const v8*)&Bs[0][wn * 64 + i * 32 + (lane & 31)][ko];
+        if (SPLIT) {
+          a1[i] = *(const v8*)&As[NOP - 1][wm * 64 + i * 32 + (lane & 31)][ko];
+          b1[i] = *(const v8*)&Bs[NOP - 1][wn * 64 + i * 32 + (lane & 31)][ko];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          acc[i][j] = Mma<BF>::op(a0[i], b0[j], acc[i][j]);
+          if (SPLIT) {
+            accx[i][j] = Mma<BF>::op(a0[i], b1[j], accx[i][j]);
+            accx[i][j] = Mma<BF>::op(a1[i], b0[j], accx[i][j]);
+          }
+        }
+    }
+    __syncthreads();
+  }
+
+  const float inv = 1.0f / (pow2_scale(*maxA) * pow2_scale(*maxB));
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int row = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (row < M && col < N) {
+          float prod = acc[i][j][e];
+          if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
+          C[(int64_t)row * ldc + col] = cpre[i][j][e] - prod * inv;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of the 128 x 128 diagonal blocks (unit-lower L or upper U), one
+// workgroup per block, for the blocked triangular solves of the refinement.
+// ---------------------------------------------------------------------------
+constexpr int TB = 128;
+
+template <int UPPER>
+__global__ void __launch_bounds__(256) tri_inv_kernel(int n, const float* __restrict__ F, int ldf,
+                                                      float* __restrict__ inv) {
+  __shared__ float S[TB][TB + 1];
+  __shared__ float X[TB][TB + 1];
+  const int blk = blockIdx.x, t = threadIdx.x;
+  const int k0 = blk * TB, nb = min(TB, n - k0);
+  float* out = inv + (int64_t)blk * TB * TB;
+  for (int idx = t; idx < TB * TB; idx += 256) {
+    int i = idx / TB, c = idx % TB;
+    S[i][c] = (i < nb && c < nb) ? F[(int64_t)(k0 + i) * ldf + k0 + c] : (i == c ? 1.f : 0.f);
+  }
+  __syncthreads();
+  if (t < TB) {
+    // column c of the inverse: solve T x = e_c by substitution
+    const int c = t;
+    if (!UPPER) {
+      for (int i = 0; i < TB; i++) {
+        float s = (i == c) ? 1.f : 0.f;
+        for (int q = c; q < i; q++) s -= S[i][q] * X[q][c];
+        X[i][c] = (i < c) ? 0.f : s;  // unit diagonal
+      }
+    } else {
+      for (int i = TB - 1; i >= 0; i--) {
+        float s = (i == c) ? 1.f : 0.f;
+        for (int q = i + 1; q <= c; q++) s -= S[i][q] * X[q][c];
+        const float d = S[i][i];
+        X[i][c] = (i > c) ? 0.f : s / (d == 0.f ? kPivotFloorF : d);
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < TB * TB; idx += 256) out[idx] = X[idx / TB][idx % TB];
+}
+
+// ---------------------------------------------------------------------------
+// Blocked triangular solve with one right-hand side, one workgroup per
+// 128-row block, all resident (cooperative launch).  Block i folds in the
+// products of its row strip with each finished solution block as soon as that
+// block's flag is set, then applies its diagonal inverse and publishes.
+// ---------------------------------------------------------------------------
+constexpr long kTrsvSpin = 1L << 23;
+
+// acc[r] (r = wave + 4 q) += sum_c tile[r][c] * v[c] over a 128 x 128 tile,
+// each lane taking columns lane and lane + 64 (coalesced row reads); the
+// per-lane partial sums are reduced across the wavefront only at the end.
+__device__ __forceinline__ void tile_gemv_accum(float s[32], const float* __restrict__ tile, int64_t ld, int rows,
+                                                int cols, const float* v, int wave, int lane) {
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    const int r = wave + 4 * q;
+    if (r < rows) {
+      const float* row = tile + (int64_t)r * ld;
+      float a = 0.f;
+      if (lane < cols) a += row[lane] * v[lane];
+      if (lane + 64 < cols) a += row[lane + 64] * v[lane + 64];
+      s[q] += a;
+    }
+  }
+}
+
+template <int UPPER>
+__global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* __restrict__ F, int ldf,
+                                                           const float* __restrict__ inv, float* __restrict__ x,
+                                                           uint32_t* __restrict__ flags, uint32_t epoch,
+                                                           int* __restrict__ err) {
+  __shared__ float xs[TB];
+  __shared__ float rhs[TB];
+  __shared__ int aborted;
+  const int nblk = (n + TB - 1) / TB;
+  const int bi = UPPER ? nblk - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int k0 = bi * TB, nb = min(TB, n - k0);
+  if (t == 0) aborted = 0;
+  float s[32];
+#pragma unroll
+  for (int q = 0; q < 32; q++) s[q] = 0.f;
+  __syncthreads();
+  const int steps = UPPER ? nblk - 1 - bi : bi;
+  for (int st = 0; st < steps; st++) {
+    const int bj = UPPER ? nblk - 1 - st : st;
+    const int j0 = bj * TB, jb = min(TB, n - j0);
+    if (t == 0) {
+      long spins = 0;
+      while ((int)(__hip_atomic_load(&flags[bj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
+        if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (t < jb) xs[t] = x[j0 + t];
+    __syncthreads();
+    tile_gemv_accum(s, F + (int64_t)k0 * ldf + j0, ldf, nb, jb, xs, wave, lane);
+    __syncthreads();
+  }
+  // reduce the partial sums across the wavefront: rhs = b - L_i,<i x_<i
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    float v = s[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int r = wave + 4 * q;
+    if (lane == 0 && r < TB) rhs[r] = (r < nb) ? x[k0 + r] - v : 0.f;
+  }
+  __syncthreads();
+  // x_i = inv_ii rhs
+#pragma unroll
+  for (int q = 0; q < 32; q++) s[q] = 0.f;
+  tile_gemv_accum(s, inv + (int64_t)bi * TB * TB, TB, nb, nb, rhs, wave, lane);
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    float v = s[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int r = wave + 4 * q;
+    if (lane == 0 && r < nb) x[k0 + r] = v;
+  }
+  __syncthreads();
+  if (t == 0) {
+    __hip_atomic_store(&flags[bi], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = b - A x (one wavefront per row), and the norms the
+// stopping test needs: max |r|, max |x| per column as float bits, and on the
+// first pass the row sums of |A| (max taken on the host).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) residual_kernel(int n, const double* __restrict__ A, int lda,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ r, double* __restrict__ rowabs,
+                                                       unsigned long long* __restrict__ norms) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* a = A + (int64_t)row * lda;
+  double s = 0.0, sa = 0.0;
+  for (int c = lane; c < n; c += 64) {
+    const double av = a[c];
+    s += av * x[c];
+    if (rowabs) sa += fabs(av);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    sa += __shfl_xor(sa, off);
+  }
+  if (lane == 0) {
+    const double rv = b[row] - s;
+    r[row] = rv;
+    if (rowabs) rowabs[row] = sa;
+    const double ar = fabs(rv), ax = fabs(x[row]);
+    atomicMax(&norms[0], (unsigned long long)__double_as_longlong(ar));
+    atomicMax(&norms[1], (unsigned long long)__double_as_longlong(ax));
+    if (rowabs) atomicMax(&norms[2], (unsigned long long)__double_as_longlong(sa));
+  }
+}
+
+__global__ void zero_u64(unsigned long long* p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void zero_u32k(unsigned* p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void axpy_d(int n, const double* __restrict__ d, double* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct MpFactor {
+  int n = 0, block = 256, prec = 2;  // prec: 0 fp16, 1 bf16, 2 fp16 split, 3 fp32
+  float* F = nullptr;                // n x n FP32 factors of W^T A V
+  float* dw = nullptr;               // 2n butterfly diagonals for W, then V
+  float* dv = nullptr;
+  float* invL = nullptr;             // nblk x 128 x 128
+  float* invU = nullptr;
+  int32_t* info = nullptr;
+  void* lo_buf = nullptr;            // FP16/BF16 operand copies
+  unsigned* maxbits = nullptr;       // [2] per step
+  uint32_t* flags = nullptr;         // TRSV flags (nblk)
+  uint32_t epoch = 0;
+  bool ok = false;
+};
+
+static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
+  f.n = n;
+  f.block = block;
+  f.prec = prec;
+  const int nblk = ceil_div(n, TB);
+  size_t lo_elems = (size_t)n * block * (prec == 2 ? 4 : 2);
+  NMX_HIP(ctx, hipMalloc(&f.F, sizeof(float) * (size_t)n * n));
+  NMX_HIP(ctx, hipMalloc(&f.dw, sizeof(float) * 4 * (size_t)n + 64));
+  f.dv = f.dw + 2 * (size_t)n;
+  NMX_HIP(ctx, hipMalloc(&f.invL, sizeof(float) * (size_t)nblk * TB * TB * 2));
+  f.invU = f.invL + (size_t)nblk * TB * TB;
+  NMX_HIP(ctx, hipMalloc(&f.info, 256));
+  NMX_HIP(ctx, hipMalloc(&f.lo_buf, 2 * lo_elems + 256));
+  NMX_HIP(ctx, hipMalloc(&f.maxbits, 256));
+  NMX_HIP(ctx, hipMalloc(&f.flags, sizeof(uint32_t) * (size_t)nblk + 256));
+  hipMemsetAsync(f.flags, 0, sizeof(uint32_t) * (size_t)nblk, ctx->stream);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+static void mp_free(MpFactor& f) {
+  if (f.F) hipFree(f.F);
+  if (f.dw) hipFree(f.dw);
+  if (f.invL) hipFree(f.invL);
+  if (f.info) hipFree(f.info);
+  if (f.lo_buf) hipFree(f.lo_buf);
+  if (f.maxbits) hipFree(f.maxbits);
+  if (f.flags) hipFree(f.flags);
+  f = MpFactor();
+}
+
+int zero_info(nmx_ctx* ctx, int32_t* info);
+
+template <int BF, int SPLIT>
+static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
+  typedef typename Lo<BF>::t lt;
+  float* F = f.F;
+  const int n = f.n;
+  const float* L21 = F + (int64_t)(k + nb) * n + k;   // m x nb, ld n
+  const float* U12 = F + (int64_t)k * n + k + nb;     // nb x m, ld n
+  lt* base = (lt*)f.lo_buf;
+  const size_t op = (size_t)m * nb;
+  lt* Ah = base;
+  lt* Al = base + op;
+  lt* Bh = base + 2 * op;
+  lt* Bl = base + 3 * op;
+  zero_u32k<<<1, 64, 0, ctx->stream>>>(f.maxbits, 2);
+  const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
+  absmax_kernel<<<g1, 256, 0, ctx->stream>>>(m, nb, L21, n, f.maxbits);
+  absmax_kernel<<<g1, 256, 0, ctx->stream>>>(nb, m, U12, n, f.maxbits + 1);
+  convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, L21, n, 0, f.maxbits, Ah, Al);
+  convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, U12, n, 1, f.maxbits + 1, Bh, Bl);
+  NMX_HIP(ctx, hipGetLastError());
+  const int tiles = ceil_div(m, MT);
+  const int grid = ceil_div(tiles * tiles, kXcds) * kXcds;
+  {
+    ProfScope ps(ctx, K_MP_GEMM, 2.0 * m * (double)m * nb * (SPLIT ? 3 : 1),
+                 8.0 * (double)m * m + (double)sizeof(lt) * 2.0 * m * nb * (SPLIT ? 2 : 1));
+    mp_gemm_kernel<BF, SPLIT><<<grid, 256, 0, ctx->stream>>>(m, m, nb, Ah, Al, Bh, Bl, f.maxbits, f.maxbits + 1,
+                                                              F + (int64_t)(k + nb) * n + k + nb, n, tiles, tiles);
+    NMX_HIP(ctx, hipGetLastError());
+  }
+  return NMX_OK;
+}
+
+// diagonal block without pivoting: recursive down to 64 columns
+static int nopiv_diag(nmx_ctx* ctx, MpFactor& f, int k, int nb) {
+  const int n = f.n;
+  float* F = f.F;
+  if (nb <= 64) {
+    ProfScope ps(ctx, K_MP_PANEL, 2.0 / 3.0 * nb * nb * (double)nb, 8.0 * nb * nb);
+    getf2_nopiv_kernel<<<1, 256, 0, ctx->stream>>>(F, n, k, nb, f.info);
+    NMX_HIP(ctx, hipGetLastError());
+    return NMX_OK;
+  }
+  int half = ((nb / 2 + 63) / 64) * 64;
+  if (half >= nb) half = nb - 64;
+  const int rest = nb - half;
+  NMX_TRY(nopiv_diag(ctx, f, k, half));
+  float* A11 = F + (int64_t)k * n + k;
+  NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, half, rest, A11, n, F + (int64_t)k * n + k + half, n));
+  // L21 := A21 U11^-1 (rows solve U11^T x^T = a^T)
+  NMX_TRY(trsm_rec_rows_upper(ctx, rest, half, A11, n, F + (int64_t)(k + half) * n + k, n));
+  NMX_TRY(gemm_launch<float>(ctx, 0, 0, rest, rest, half, -1.0f, F + (int64_t)(k + half) * n + k, n,
+                             F + (int64_t)k * n + k + half, n, 1.0f, F + (int64_t)(k + half) * n + k + half, n, 0));
+  return nopiv_diag(ctx, f, k + half, rest);
+}
+
+static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed) {
+  const int n = f.n;
+  // butterflies and the transformed FP32 matrix
+  rbt_diag_kernel<<<ceil_div(4 * n, 256), 256, 0, ctx->stream>>>(4 * n, seed, f.dw);
+  const int n4 = n & ~3, h = n4 / 4;
+  const int64_t work = (int64_t)h * h + (int64_t)(n - n4) * n + (int64_t)n4 * (n - n4);
+  {
+    ProfScope ps(ctx, K_OTHER, 0.0, 12.0 * (double)n * n);
+    rbt_matrix_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, A, lda, f.F, n, f.dw, f.dv);
+    NMX_HIP(ctx, hipGetLastError());
+  }
+  NMX_TRY(zero_info(ctx, f.info));
+  for (int k = 0; k < n; k += f.block) {
+    const int nb = std::min(f.block, n - k);
+    NMX_TRY(nopiv_diag(ctx, f, k, nb));
+    const int m = n - k - nb;
+    if (m <= 0) break;
+    float* A11 = f.F + (int64_t)k * n + k;
+    NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, nb, m, A11, n, f.F + (int64_t)k * n + k + nb, n));
+    NMX_TRY(trsm_rec_rows_upper(ctx, m, nb, A11, n, f.F + (int64_t)(k + nb) * n + k, n));
+    switch (f.prec) {
+      case 0: NMX_TRY((mp_update<0, 0>(ctx, f, k, nb, m))); break;
+      case 1: NMX_TRY((mp_update<1, 0>(ctx, f, k, nb, m))); break;
+      case 2: NMX_TRY((mp_update<0, 1>(ctx, f, k, nb, m))); break;
+      default:
+        NMX_TRY(gemm_launch<float>(ctx, 0, 0, m, m, nb, -1.0f, f.F + (int64_t)(k + nb) * n + k, n,
+                                   f.F + (int64_t)k * n + k + nb, n, 1.0f, f.F + (int64_t)(k + nb) * n + k + nb, n, 0));
+    }
+  }
+  const int nblk = ceil_div(n, TB);
+  tri_inv_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL);
+  tri_inv_kernel<1><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invU);
+  NMX_HIP(ctx, hipGetLastError());
+  f.ok = true;
+  return NMX_OK;
+}
+
+// d (FP64, n) = V U^-1 L^-1 W^T r  (accumulate: x += d)
+static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, float* tmp, int accumulate) {
+  const int n = f.n, n4 = n & ~3, h = n4 / 4;
+  const int64_t work = (int64_t)h + (n - n4);
+  rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, r, 1, tmp, 1, f.dw);
+  NMX_HIP(ctx, hipGetLastError());
+  const int nblk = ceil_div(n, TB);
+  for (int up = 0; up < 2; up++) {
+    f.epoch += 1;
+    uint32_t ep = f.epoch;
+    const float* F = f.F;
+    const float* inv = up ? f.invU : f.invL;
+    int nn = n, ld = n;
+    uint32_t* flags = f.flags;
+    int* err = ctx->err;
+    void* args[] = {&nn, (void*)&F, &ld, (void*)&inv, &tmp, &flags, &ep, &err};
+    ProfScope ps(ctx, K_MP_TRSV, 2.0 * (double)n * n / 2, 4.0 * (double)n * n / 2);
+    const void* fn = up ? (const void*)trsv_blocked_kernel<1> : (const void*)trsv_blocked_kernel<0>;
+    NMX_HIP(ctx, hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, ctx->stream));
+  }
+  rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, tmp, 1, x, 1, f.dv,
+                                                                                 accumulate);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+static int residual(nmx_ctx* ctx, int n, const double* A, int lda, const double* x, const double* b, double* r,
+                    double* rowabs, unsigned long long* norms, double out[3]) {
+  zero_u64<<<1, 64, 0, ctx->stream>>>(norms, 3);
+  {
+    ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n + 24.0 * n);
+    residual_kernel<<<ceil_div(n, 4), 256, 0, ctx->stream>>>(n, A, lda, x, b, r, rowabs, norms);
+    NMX_HIP(ctx, hipGetLastError());
+  }
+  unsigned long long h[3];
+  NMX_HIP(ctx, hipMemcpyAsync(h, norms, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < 3; i++) memcpy(&out[i], &h[i], 8);
+  return NMX_OK;
+}
+
+// ---- C ABI -------------------------------------------------------------------
+NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x,
+                          const nmx_ir_cfg* cfg_in, nmx_ir_report* rep) {
+  if (!ctx) return -1;
+  if (n <= 0) return -2;
+  if (!A) return -3;
+  if (lda < n) return -4;
+  if (!b) return -5;
+  if (!x) return -6;
+  nmx_ir_cfg cfg;
+  if (cfg_in) {
+    cfg = *cfg_in;
+  } else {
+    nmx_ir_cfg_default(&cfg);
+  }
+  if (cfg.precision < 0 || cfg.precision > 3) return -7;
+  if (cfg.block <= 0) cfg.block = 256;
+  if (cfg.block % 64) return -7;
+  if (n > 65536) return NMX_ERR_UNSUPPORTED;
+  nmx_ir_report local{};
+  nmx_ir_report* R = rep ? rep : &local;
+  *R = nmx_ir_report{};
+
+  MpFactor f;
+  int st = mp_alloc(ctx, f, n, cfg.block, cfg.precision);
+  void* wsp = nullptr;
+  if (st == NMX_OK) st = nmx_workspace(ctx, sizeof(double) * (3 * (size_t)n + 8) + sizeof(float) * (size_t)n, &wsp);
+  if (st != NMX_OK) {
+    mp_free(f);
+    return st;
+  }
+  double* r = (double*)wsp;
+  double* rowabs = r + n;
+  double* d = rowabs + n;
+  unsigned long long* norms = (unsigned long long*)(d + n);
+  float* tmp = (float*)(norms + 8);
+
+  hipEvent_t e0, e1, e2;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventCreate(&e2);
+  hipEventRecord(e0, ctx->stream);
+  st = mp_factor(ctx, f, A, lda, cfg.seed);
+  hipEventRecord(e1, ctx->stream);
+  if (st == NMX_OK) {
+    // x0 = M^-1 b
+    st = mp_solve_vec(ctx, f, b, x, tmp, 0);
+  }
+  double nrm[3] = {0, 0, 0};
+  double anorm = 0.0;
+  int it = 0;
+  double eta = INFINITY, prev = INFINITY;
+  while (st == NMX_OK) {
+    st = residual(ctx, n, A, lda, x, b, r, it == 0 ? rowabs : nullptr, norms, nrm);
+    if (st != NMX_OK) break;
+    if (it == 0) anorm = nrm[2];
+    eta = nrm[0] / (anorm * nrm[1]);
+    R->history[it < 32 ? it : 31] = eta;
+    if (!(eta > cfg.tol) || it >= cfg.max_iter) break;
+    if (it > 0 && !(eta < 0.5 * prev)) break;  // stagnation
+    prev = eta;
+    st = mp_solve_vec(ctx, f, r, x, tmp, 1);
+    it++;
+  }
+  hipEventRecord(e2, ctx->stream);
+  hipEventSynchronize(e2);
+  float fms = 0.f, rms = 0.f;
+  hipEventElapsedTime(&fms, e0, e1);
+  hipEventElapsedTime(&rms, e1, e2);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipEventDestroy(e2);
+  int32_t hinfo = 0;
+  hipMemcpy(&hinfo, f.info, sizeof(int32_t), hipMemcpyDeviceToHost);
+  R->iterations = it;
+  R->backward_error = eta;
+  R->converged = (eta <= cfg.tol) ? 1 : 0;
+  R->factor_ms = fms;
+  R->refine_ms = rms;
+  R->info = hinfo;
+  mp_free(f);
+  if (st != NMX_OK) return st;
+  int dev = nmx_ctx_check(ctx);
+  if (dev != NMX_OK) return dev;
+  return R->converged ? NMX_OK : NMX_ERR_NOT_CONVERGED;
+}
+
+NMX_API void nmx_ir_cfg_default(nmx_ir_cfg* cfg) {
+  if (!cfg) return;
+  cfg->precision = 2;
+  cfg->block = 256;
+  cfg->max_iter = 30;
+  cfg->solver = 0;
+  cfg->gmres_restart = 30;
+  cfg->tol = 1e-15;
+  cfg->seed = 0x5eedULL;
+}

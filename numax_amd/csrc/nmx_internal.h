@@ -1,0 +1,131 @@
+// Internal declarations shared by the HIP translation units of libnumax_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/numax_hip.h"
+
+#define NMX_API extern "C" __attribute__((visibility("default")))
+
+// Return on a failing HIP call with NMX_ERR_HIP (the failing call's error is
+// kept in ctx->last_hip for diagnostics).
+#define NMX_HIP(ctx, call)                          \
+  do {                                              \
+    hipError_t e_ = (call);                         \
+    if (e_ != hipSuccess) {                         \
+      if (ctx) (ctx)->last_hip = (int)e_;           \
+      return NMX_ERR_HIP;                           \
+    }                                               \
+  } while (0)
+
+#define NMX_TRY(expr)           \
+  do {                          \
+    int s_ = (expr);            \
+    if (s_ != NMX_OK) return s_; \
+  } while (0)
+
+// Leaf width of the pivoting panel and the cross-workgroup mailbox shape.
+constexpr int kLeafW = 32;           // columns one cooperative leaf factors
+constexpr int kLeafThreads = 1024;   // threads per participating workgroup
+constexpr int kMaxParts = 32;        // participating workgroups at most
+constexpr int kXcds = 8;             // MI355X accelerator complex dies
+
+// Kernel classes timed by the context's optional event profiler.
+enum NmxKernel {
+  K_GEMM = 0,     // MFMA GEMM update (FP64 / FP32)
+  K_LEAF,         // cooperative pivoting leaf
+  K_LASWP,        // row interchanges
+  K_TRSM,         // triangular-solve base blocks
+  K_POTRF_DIAG,   // Cholesky diagonal block
+  K_MP_GEMM,      // mixed-precision (FP16/BF16 MFMA) trailing update
+  K_MP_PANEL,     // mixed-precision diagonal block / panel
+  K_RESIDUAL,     // FP64 residual b - A x
+  K_MP_TRSV,      // low-precision triangular solves inside refinement
+  K_OTHER,
+  K_COUNT
+};
+
+struct ProfRec {
+  hipEvent_t a, b;
+  int kind;
+  double flops, bytes;
+};
+
+struct nmx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 0;
+  int last_hip = 0;
+  // grow-only scratch
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  // cross-workgroup mailbox for the cooperative panel: flags then slots
+  void* mailbox = nullptr;
+  size_t mailbox_bytes = 0;
+  uint32_t epoch = 0;  // last epoch handed to a kernel (monotonic)
+  // device error word (sticky): NMX_ERR_SYNC_TIMEOUT from bounded waits
+  int* err = nullptr;
+  // event profiler (off unless nmx_ctx_set_profiling)
+  bool profiling = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  int64_t prof_launches[K_COUNT] = {};
+  double prof_ms[K_COUNT] = {}, prof_flops[K_COUNT] = {}, prof_bytes[K_COUNT] = {};
+};
+
+// Brackets one launch with events when the context is profiling.
+struct ProfScope {
+  nmx_ctx* ctx;
+  ProfRec rec;
+  bool on;
+  ProfScope(nmx_ctx* c, int kind, double flops, double bytes);
+  ~ProfScope();
+};
+
+// Workspace of at least `bytes`, 256-byte aligned; grows (with a stream sync)
+// when needed.
+int nmx_workspace(nmx_ctx* ctx, size_t bytes, void** out);
+
+// Mailbox sized for `parts` participants with `w` values of `elem` bytes.
+int nmx_mailbox(nmx_ctx* ctx, void** out);
+constexpr size_t kMailboxFlagsBytes = kMaxParts * 64;                  // one flag per 64 B line
+constexpr size_t kMailboxSlotBytes = 64 + kLeafW * sizeof(double);    // mag,row + a row of values
+constexpr size_t kMailboxBytes = kMailboxFlagsBytes + 2 * (kMaxParts + 1) * kMailboxSlotBytes;
+
+// Advance and return the epoch base for `columns` upcoming columns; resets
+// the flags (with a fill kernel) before the 32-bit counter could wrap.
+int nmx_epoch_reserve(nmx_ctx* ctx, uint32_t columns, uint32_t* base);
+
+// ---- templated building blocks (instantiated for float and double) --------
+template <typename T>
+int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alpha, const T* A, int lda, const T* B,
+                int ldb, T beta, T* C, int ldc, int lower_only = 0);
+
+// Left-side triangular solve op(T) X = B (vectors are columns of B).
+template <typename T>
+int trsm_left(nmx_ctx* ctx, int upper, int unit, int trans, int n, int nrhs, const T* Tm, int ldt, T* B, int ldb);
+
+// Right-side X := X L^-T with L lower n x n; X is m x n (the Cholesky panel).
+template <typename T>
+int trsm_right_lower_t(nmx_ctx* ctx, int m, int n, const T* L, int ldl, T* X, int ldx);
+
+int trsm_rec_rows_upper(nmx_ctx* ctx, int m, int n, const float* U, int ldu, float* X, int ldx);
+
+template <typename T>
+int laswp_launch(nmx_ctx* ctx, T* A, int lda, int col_lo, int col_hi, int skip_lo, int skip_hi, const int32_t* ipiv,
+                 int k1, int k2);
+
+template <typename T>
+int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info);
+
+template <typename T>
+int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
+
+template <typename T>
+int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }

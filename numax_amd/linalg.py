@@ -1,0 +1,231 @@
+"""Host-side mirror of numax.linalg's dense factor-and-solve surface.
+
+Same names, argument meaning and error behaviour as numax's Tensor tier
+(numax/linalg/lu.mojo, cholesky.mojo, triangular.mojo, basic.mojo), with the
+device work done by libnumax_hip through its C ABI:
+
+* ``lu_factor(a)`` -> ``LU`` (``.factored``, ``.pivots``, ``.sign``, ``.solve``,
+  ``.det``, ``.slogdet``, ``.singular``)            -- lu.mojo:75-701, :998-1361
+* ``solve(a, b)``                                    -- basic.mojo:55-146
+* ``cholesky(a)``, ``cholesky_solve(lower, b)``      -- cholesky.mojo:233-580
+* ``solve_triangular(a, b, upper, unit, trans)``     -- triangular.mojo:228-425
+
+Arrays are ``torch`` tensors on a ROCm device (row-major, contiguous); NumPy
+inputs are copied to ``cuda:0``.  Inputs are never modified (the reference
+copies ``a`` into a work tensor too, lu.mojo:593).  There is no CPU path:
+without the built library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+__all__ = ["LU", "NumaxError", "lu_factor", "solve", "cholesky", "cholesky_solve", "solve_triangular", "gemm"]
+
+
+class NumaxError(ValueError):
+    """The reference's raised ``Error`` (shape problems, non-PD Cholesky)."""
+
+
+_SUFFIX = {torch.float64: "f64", torch.float32: "f32"}
+
+
+def _device_tensor(x, dtype=None) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        t = x
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x))
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    if t.device.type != "cuda":
+        t = t.to("cuda:0")
+    return t.contiguous()
+
+
+def _ctx(t: torch.Tensor) -> _lib.Context:
+    dev = t.device.index or 0
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    return _lib.context_for(dev, stream)
+
+
+def _ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _fn(name: str, dtype: torch.dtype):
+    if dtype not in _SUFFIX:
+        raise NumaxError(f"{name}: dtype {dtype} is not a supported floating-point type (float32, float64)")
+    return getattr(_lib.load(), f"nmx_{name}_{_SUFFIX[dtype]}")
+
+
+def _square(a: torch.Tensor, who: str) -> int:
+    if a.dim() != 2 or a.shape[0] != a.shape[1]:
+        raise NumaxError(f"{who}: matrix must be square, got shape {tuple(a.shape)}")
+    return int(a.shape[0])
+
+
+class LU:
+    """A blocked LU factorization with partial pivoting (numax ``LU`` /
+    ``DynamicLU``, lu.mojo:75-451, :998-1197).
+
+    ``factored`` holds L strictly below the diagonal (unit diagonal implicit)
+    and U on and above it; ``pivots[j]`` is the row column j interchanged with
+    (LAPACK ipiv, 0-based); ``sign`` is the interchange parity.
+    """
+
+    def __init__(self, factored: torch.Tensor, pivots: torch.Tensor, sign: int, info: int):
+        self.factored = factored
+        self.pivots = pivots
+        self.sign = sign
+        self.info = info
+
+    @property
+    def n(self) -> int:
+        return int(self.factored.shape[0])
+
+    def singular(self) -> bool:
+        """``DynamicLU.singular`` (lu.mojo:1047-1056): an exactly zero pivot
+        was met (it was divided as 1e-30, not raised)."""
+        return self.info != 0
+
+    def solve(self, b) -> torch.Tensor:
+        """``x`` with ``A x = b`` (a vector) or ``X`` with ``A X = B`` (an
+        ``n x k`` matrix), reusing this factorization (lu.mojo:181-354)."""
+        x = _device_tensor(b, self.factored.dtype).clone()
+        n = self.n
+        if x.dim() == 1:
+            if x.shape[0] != n:
+                raise NumaxError(f"lu_solve: b has {x.shape[0]} entries for an order-{n} factorization")
+            nrhs, ldb = 1, 1
+        elif x.dim() == 2:
+            if x.shape[0] != n:
+                raise NumaxError(f"lu_solve: b has {x.shape[0]} rows for an order-{n} factorization")
+            nrhs = int(x.shape[1])
+            ldb = max(nrhs, 1)
+        else:
+            raise NumaxError("lu_solve: b must be a vector or a matrix")
+        ctx = _ctx(self.factored)
+        _lib.check(_fn("getrs", self.factored.dtype)(ctx.handle, n, nrhs, _ptr(self.factored), n,
+                                                       _ptr(self.pivots), _ptr(x), ldb), "lu_solve")
+        return x
+
+    def _diagonal(self) -> np.ndarray:
+        d = torch.empty(self.n, dtype=self.factored.dtype, device=self.factored.device)
+        ctx = _ctx(self.factored)
+        _lib.check(_fn("diag", self.factored.dtype)(ctx.handle, self.n, _ptr(self.factored), self.n, _ptr(d)),
+                   "det")
+        return d.cpu().numpy()
+
+    def det(self) -> float:
+        """Product of U's diagonal times the swap parity (lu.mojo:382-404)."""
+        prod = float(self.sign)
+        for v in self._diagonal():
+            prod *= float(v)
+        return prod
+
+    def slogdet(self) -> tuple[float, float]:
+        """``(sign, ln|det|)``; ``(0, -inf)`` on a zero diagonal entry
+        (lu.mojo:406-451)."""
+        sign = float(self.sign)
+        total = 0.0
+        for v in self._diagonal():
+            v = float(v)
+            if v == 0.0:
+                return 0.0, -math.inf
+            if v < 0:
+                sign = -sign
+            total += math.log(abs(v))
+        return sign, total
+
+
+def lu_factor(a, block: int = 256, base: int = 32) -> LU:
+    """Factor ``a`` into ``P L U`` (numax ``lu_factor``, lu.mojo:454-701).
+
+    ``block`` is the panel width and ``base`` the recursive panel's leaf width
+    (the reference's parameters of the same names); a singular matrix is not
+    reported as an error (see ``LU.singular``).
+    """
+    work = _device_tensor(a).clone()
+    n = _square(work, "lu_factor")
+    piv = torch.zeros(max(n, 1), dtype=torch.int32, device=work.device)
+    info = torch.zeros(1, dtype=torch.int32, device=work.device)
+    ctx = _ctx(work)
+    _lib.check(_fn("getrf", work.dtype)(ctx.handle, n, _ptr(work), max(n, 1), _ptr(piv), _ptr(info), block, base),
+               "lu_factor")
+    host_piv = piv[:n].cpu().numpy()
+    ctx.check_device()
+    sign = 1
+    for j, p in enumerate(host_piv):
+        if p != j:
+            sign = -sign
+    return LU(work, piv[:n], sign, int(info.item()))
+
+
+def solve(a, b, block: int = 256) -> torch.Tensor:
+    """``lu_factor`` then the solve at the same device (basic.mojo:55-146)."""
+    return lu_factor(a, block=block).solve(b)
+
+
+def cholesky(a, block: int = 256) -> torch.Tensor:
+    """Lower ``L`` with ``L L^T = a`` (cholesky.mojo:233-396).  Reads only the
+    lower triangle; raises naming the 0-based index of the first non-positive
+    pivot, after the whole factorization has run."""
+    work = _device_tensor(a).clone()
+    n = _square(work, "cholesky")
+    info = torch.zeros(1, dtype=torch.int32, device=work.device)
+    ctx = _ctx(work)
+    _lib.check(_fn("potrf", work.dtype)(ctx.handle, n, _ptr(work), max(n, 1), _ptr(info), block), "cholesky")
+    flag = int(info.item())
+    if flag != 0:
+        raise NumaxError(f"cholesky: matrix is not positive definite (non-positive pivot at index {flag - 1})")
+    return work
+
+
+def cholesky_solve(lower, b) -> torch.Tensor:
+    """``x`` with ``L L^T x = b`` from the Cholesky factor (cholesky.mojo:471-580)."""
+    l = _device_tensor(lower)
+    n = _square(l, "cholesky_solve")
+    x = _device_tensor(b, l.dtype).clone()
+    if x.shape[0] != n:
+        raise NumaxError(f"cholesky_solve: b has {x.shape[0]} entries for an order-{n} factor")
+    nrhs = 1 if x.dim() == 1 else int(x.shape[1])
+    ctx = _ctx(l)
+    _lib.check(_fn("potrs", l.dtype)(ctx.handle, n, nrhs, _ptr(l), n, _ptr(x), max(nrhs, 1)), "cholesky_solve")
+    return x
+
+
+def solve_triangular(a, b, upper: bool = False, unit: bool = False, trans: bool = False) -> torch.Tensor:
+    """``solve_triangular`` (triangular.mojo:228-425): ``upper`` names the
+    stored triangle (SciPy's ``lower`` inverted), ``trans`` solves against
+    ``a^T`` without transposing, ``unit`` ignores the diagonal."""
+    t = _device_tensor(a)
+    n = _square(t, "solve_triangular")
+    x = _device_tensor(b, t.dtype).clone()
+    if x.shape[0] != n:
+        raise NumaxError(f"solve_triangular: b has {x.shape[0]} entries for an order-{n} matrix")
+    nrhs = 1 if x.dim() == 1 else int(x.shape[1])
+    walk_upper = bool(upper) != bool(trans)
+    ctx = _ctx(t)
+    _lib.check(_fn("trsm", t.dtype)(ctx.handle, int(walk_upper), int(bool(unit)), int(bool(trans)), n, nrhs, _ptr(t),
+                                    n, _ptr(x), max(nrhs, 1)), "solve_triangular")
+    return x
+
+
+def gemm(a, b, c=None, alpha: float = 1.0, beta: float = 0.0, transa: bool = False, transb: bool = False):
+    """``c = beta c + alpha op(a) op(b)`` on the MFMA kernel the drivers use."""
+    A = _device_tensor(a)
+    B = _device_tensor(b, A.dtype)
+    m, k = (A.shape[1], A.shape[0]) if transa else (A.shape[0], A.shape[1])
+    kb, n = (B.shape[1], B.shape[0]) if transb else (B.shape[0], B.shape[1])
+    if k != kb:
+        raise NumaxError(f"gemm: inner dimensions {k} and {kb} differ")
+    C = torch.zeros((m, n), dtype=A.dtype, device=A.device) if c is None else _device_tensor(c, A.dtype).clone()
+    ctx = _ctx(A)
+    _lib.check(_fn("gemm", A.dtype)(ctx.handle, int(transa), int(transb), m, n, k, alpha, _ptr(A), A.shape[1],
+                                    _ptr(B), B.shape[1], beta, _ptr(C), max(n, 1)), "gemm")
+    return C
