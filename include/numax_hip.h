@@ -123,7 +123,8 @@ typedef struct nmx_ir_cfg {
   int solver;        /* 0 auto, 1 classical refinement, 2 GMRES-IR */
   int gmres_restart; /* GMRES restart length */
   double tol;        /* target backward error */
-  uint64_t seed;     /* butterfly seed */
+  uint64_t seed;     /* butterfly seed (pivot = 0) */
+  int pivot;         /* 1: partial pivoting (default); 0: random butterflies and no pivoting */
 } nmx_ir_cfg;
 
 typedef struct nmx_ir_report {
@@ -140,6 +141,12 @@ typedef struct nmx_ir_report {
 void nmx_ir_cfg_default(nmx_ir_cfg* cfg);
 int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x, const nmx_ir_cfg* cfg,
                   nmx_ir_report* report);
+/* Test hook of the factor stage: stage 0 returns W^T A V, stage 1 its FP32
+ * no-pivot L\U; `diags` receives the 4n butterfly diagonals (W's r0, r1, s0,
+ * s1, t0, t1 then V's), F is n x n row-major (device), info the first zero
+ * pivot. */
+int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, int precision, int block, uint64_t seed,
+                        int stage, int pivot, float* F, float* diags, int32_t* info, int32_t* ipiv);
 
 /* ---- synthetic inputs ------------------------------------------------------
  * Philox4x32-10 stream (the same words as numax_amd.random): kind 0 fills

@@ -52,7 +52,24 @@ SIGNATURES = {
     "nmx_gemm_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_flt, _c_vp, _c_int, _c_vp,
                               _c_int, _c_flt, _c_vp, _c_int]),
     "nmx_fill_random_f64": (_c_int, [_c_vp, _c_int, _c_u64, _c_i64, _c_i64, _c_dbl, _c_dbl, _c_vp]),
+    "nmx_ir_cfg_default": (None, [_c_vp]),
+    "nmx_gesv_mpir": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nmx_mp_factor_debug": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_int, _c_int, _c_u64, _c_int, _c_int,
+                                     _c_vp, _c_vp, _c_vp, _c_vp]),
 }
+
+
+class IrCfg(ctypes.Structure):
+    """``nmx_ir_cfg`` (include/numax_hip.h)."""
+    _fields_ = [("precision", _c_int), ("block", _c_int), ("max_iter", _c_int), ("solver", _c_int),
+                ("gmres_restart", _c_int), ("tol", _c_dbl), ("seed", _c_u64), ("pivot", _c_int)]
+
+
+class IrReport(ctypes.Structure):
+    """``nmx_ir_report`` (include/numax_hip.h)."""
+    _fields_ = [("iterations", _c_int), ("gmres_iterations", _c_int), ("converged", _c_int), ("info", _c_int),
+                ("backward_error", _c_dbl), ("factor_ms", _c_dbl), ("refine_ms", _c_dbl),
+                ("history", _c_dbl * 32)]
 
 _lib = None
 _lock = threading.Lock()

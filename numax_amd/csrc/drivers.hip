@@ -37,8 +37,12 @@ int panel_rec(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, 
 }  // namespace
 
 // lu_factor (numax/linalg/lu.mojo:454-701): right-looking blocked GEPP.
+// `update`, when given, replaces the FP64/FP32 GEMM of the block-level
+// trailing update A22 -= L21 U12 (the mixed-precision factorization plugs
+// its FP16 / BF16 MFMA update in here).
 template <typename T>
-int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base) {
+int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base,
+                  TrailingUpdate update, void* user) {
   if (block <= 0) block = 256;
   if (base <= 0 || base > kLeafW) base = kLeafW;
   NMX_TRY(zero_info(ctx, info));
@@ -49,12 +53,22 @@ int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info,
     if (m > 0) {
       // U12 := L11^-1 A12 over the whole block row, then A22 -= L21 U12.
       NMX_TRY(trsm_left<T>(ctx, 0, 1, 0, nb, m, AT(A, lda, k, k), lda, AT(A, lda, k, k + nb), lda));
-      NMX_TRY(gemm_launch<T>(ctx, 0, 0, m, m, nb, T(-1), AT(A, lda, k + nb, k), lda, AT(A, lda, k, k + nb), lda,
-                             T(1), AT(A, lda, k + nb, k + nb), lda, 0));
+      if (update) {
+        NMX_TRY(update(ctx, user, k, nb, m));
+      } else {
+        NMX_TRY(gemm_launch<T>(ctx, 0, 0, m, m, nb, T(-1), AT(A, lda, k + nb, k), lda, AT(A, lda, k, k + nb), lda,
+                               T(1), AT(A, lda, k + nb, k + nb), lda, 0));
+      }
     }
   }
   return NMX_OK;
 }
+
+template <typename T>
+int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base) {
+  return getrf_impl_ex<T>(ctx, n, A, lda, ipiv, info, block, base, nullptr, nullptr);
+}
+template int getrf_impl_ex<float>(nmx_ctx*, int, float*, int, int32_t*, int32_t*, int, int, TrailingUpdate, void*);
 
 // _potrf_blocked (numax/linalg/cholesky.mojo:59-230): diagonal block, panel
 // solve X := X L11^-T, and the lower block triangle of A22 -= L21 L21^T.

@@ -619,9 +619,21 @@ __global__ void zero_u32k(unsigned* p, int n) {
   if (i < n) p[i] = 0u;
 }
 
-__global__ void axpy_d(int n, const double* __restrict__ d, double* __restrict__ x) {
+__global__ void cast_f64_f32(int n, const double* __restrict__ A, int lda, float* __restrict__ F, int ldf) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  F[(int64_t)r * ldf + c] = (float)A[(int64_t)r * lda + c];
+}
+
+__global__ void vec_f64_f32(int n, const double* __restrict__ v, float* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += d[i];
+  if (i < n) out[i] = (float)v[i];
+}
+
+__global__ void vec_f32_add_f64(int n, const float* __restrict__ y, double* __restrict__ x, int accumulate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = accumulate ? x[i] + (double)y[i] : (double)y[i];
 }
 
 }  // namespace
@@ -631,6 +643,8 @@ __global__ void axpy_d(int n, const double* __restrict__ d, double* __restrict__
 // ===========================================================================
 struct MpFactor {
   int n = 0, block = 256, prec = 2;  // prec: 0 fp16, 1 bf16, 2 fp16 split, 3 fp32
+  int pivot = 1;                     // 1: partial pivoting (GEPP); 0: butterflies, no pivoting
+  int32_t* ipiv = nullptr;
   float* F = nullptr;                // n x n FP32 factors of W^T A V
   float* dw = nullptr;               // 2n butterfly diagonals for W, then V
   float* dv = nullptr;
@@ -649,13 +663,14 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.block = block;
   f.prec = prec;
   const int nblk = ceil_div(n, TB);
-  size_t lo_elems = (size_t)n * block * (prec == 2 ? 4 : 2);
+  size_t lo_elems = (size_t)n * block * 4;  // A_hi, A_lo, B_hi, B_lo (lo unused unless split)
   NMX_HIP(ctx, hipMalloc(&f.F, sizeof(float) * (size_t)n * n));
   NMX_HIP(ctx, hipMalloc(&f.dw, sizeof(float) * 4 * (size_t)n + 64));
   f.dv = f.dw + 2 * (size_t)n;
   NMX_HIP(ctx, hipMalloc(&f.invL, sizeof(float) * (size_t)nblk * TB * TB * 2));
   f.invU = f.invL + (size_t)nblk * TB * TB;
   NMX_HIP(ctx, hipMalloc(&f.info, 256));
+  NMX_HIP(ctx, hipMalloc(&f.ipiv, sizeof(int32_t) * (size_t)n + 256));
   NMX_HIP(ctx, hipMalloc(&f.lo_buf, 2 * lo_elems + 256));
   NMX_HIP(ctx, hipMalloc(&f.maxbits, 256));
   NMX_HIP(ctx, hipMalloc(&f.flags, sizeof(uint32_t) * (size_t)nblk + 256));
@@ -669,6 +684,7 @@ static void mp_free(MpFactor& f) {
   if (f.dw) hipFree(f.dw);
   if (f.invL) hipFree(f.invL);
   if (f.info) hipFree(f.info);
+  if (f.ipiv) hipFree(f.ipiv);
   if (f.lo_buf) hipFree(f.lo_buf);
   if (f.maxbits) hipFree(f.maxbits);
   if (f.flags) hipFree(f.flags);
@@ -732,8 +748,36 @@ static int nopiv_diag(nmx_ctx* ctx, MpFactor& f, int k, int nb) {
   return nopiv_diag(ctx, f, k + half, rest);
 }
 
-static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed) {
+static int mp_update_hook(nmx_ctx* ctx, void* user, int k, int nb, int m) {
+  MpFactor& f = *(MpFactor*)user;
+  switch (f.prec) {
+    case 0: return mp_update<0, 0>(ctx, f, k, nb, m);
+    case 1: return mp_update<1, 0>(ctx, f, k, nb, m);
+    default: return mp_update<0, 1>(ctx, f, k, nb, m);
+  }
+}
+
+static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed, int stop_after_transform = 0) {
   const int n = f.n;
+  if (f.pivot) {
+    // GEPP in FP32 (the cooperative pivoting panel), the block-level trailing
+    // update on FP16 / BF16 MFMA
+    const int64_t total = (int64_t)n * n;
+    {
+      ProfScope ps(ctx, K_OTHER, 0.0, 12.0 * (double)total);
+      cast_f64_f32<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(n, A, lda, f.F, n);
+      NMX_HIP(ctx, hipGetLastError());
+    }
+    if (stop_after_transform) return zero_info(ctx, f.info);
+    NMX_TRY(getrf_impl_ex<float>(ctx, n, f.F, n, f.ipiv, f.info, f.block, kLeafW,
+                                 f.prec == 3 ? nullptr : mp_update_hook, &f));
+    const int nblk = ceil_div(n, TB);
+    tri_inv_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL);
+    tri_inv_kernel<1><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invU);
+    NMX_HIP(ctx, hipGetLastError());
+    f.ok = true;
+    return NMX_OK;
+  }
   // butterflies and the transformed FP32 matrix
   rbt_diag_kernel<<<ceil_div(4 * n, 256), 256, 0, ctx->stream>>>(4 * n, seed, f.dw);
   const int n4 = n & ~3, h = n4 / 4;
@@ -744,6 +788,7 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
     NMX_HIP(ctx, hipGetLastError());
   }
   NMX_TRY(zero_info(ctx, f.info));
+  if (stop_after_transform) return NMX_OK;
   for (int k = 0; k < n; k += f.block) {
     const int nb = std::min(f.block, n - k);
     NMX_TRY(nopiv_diag(ctx, f, k, nb));
@@ -773,8 +818,14 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
 static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, float* tmp, int accumulate) {
   const int n = f.n, n4 = n & ~3, h = n4 / 4;
   const int64_t work = (int64_t)h + (n - n4);
-  rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, r, 1, tmp, 1, f.dw);
-  NMX_HIP(ctx, hipGetLastError());
+  if (f.pivot) {
+    vec_f64_f32<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, r, tmp);
+    NMX_HIP(ctx, hipGetLastError());
+    NMX_TRY(laswp_launch<float>(ctx, tmp, 1, 0, 1, 0, 0, f.ipiv, 0, n));
+  } else {
+    rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, r, 1, tmp, 1, f.dw);
+    NMX_HIP(ctx, hipGetLastError());
+  }
   const int nblk = ceil_div(n, TB);
   for (int up = 0; up < 2; up++) {
     f.epoch += 1;
@@ -789,8 +840,12 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
     const void* fn = up ? (const void*)trsv_blocked_kernel<1> : (const void*)trsv_blocked_kernel<0>;
     NMX_HIP(ctx, hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, ctx->stream));
   }
-  rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, tmp, 1, x, 1, f.dv,
-                                                                                 accumulate);
+  if (f.pivot) {
+    vec_f32_add_f64<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, tmp, x, accumulate);
+  } else {
+    rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, tmp, 1, x, 1, f.dv,
+                                                                                   accumulate);
+  }
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
@@ -834,7 +889,9 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   *R = nmx_ir_report{};
 
   MpFactor f;
+  f.pivot = cfg.pivot;
   int st = mp_alloc(ctx, f, n, cfg.block, cfg.precision);
+  f.pivot = cfg.pivot;
   void* wsp = nullptr;
   if (st == NMX_OK) st = nmx_workspace(ctx, sizeof(double) * (3 * (size_t)n + 8) + sizeof(float) * (size_t)n, &wsp);
   if (st != NMX_OK) {
@@ -906,4 +963,30 @@ NMX_API void nmx_ir_cfg_default(nmx_ir_cfg* cfg) {
   cfg->gmres_restart = 30;
   cfg->tol = 1e-15;
   cfg->seed = 0x5eedULL;
+  cfg->pivot = 1;
+}
+
+// Test hook: the transformed matrix (stage 0) or its no-pivot factors
+// (stage 1) copied to F (n x n FP32, ld n), the 4n butterfly diagonals (W's
+// 2n then V's 2n) to diags.
+NMX_API int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, int precision, int block, uint64_t seed,
+                                int stage, int pivot, float* F, float* diags, int32_t* info, int32_t* ipiv) {
+  if (!ctx) return -1;
+  if (n <= 0) return -2;
+  if (lda < n) return -4;
+  if (precision < 0 || precision > 3) return -5;
+  if (block <= 0 || block % 64) return -6;
+  MpFactor f;
+  int st = mp_alloc(ctx, f, n, block, precision);
+  f.pivot = pivot;
+  if (st == NMX_OK) st = mp_factor(ctx, f, A, lda, seed, stage == 0);
+  if (st == NMX_OK) {
+    hipMemcpyAsync(F, f.F, sizeof(float) * (size_t)n * n, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(diags, f.dw, sizeof(float) * 4 * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(info, f.info, sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream);
+    if (ipiv) hipMemcpyAsync(ipiv, f.ipiv, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = NMX_ERR_HIP;
+  }
+  mp_free(f);
+  return st;
 }

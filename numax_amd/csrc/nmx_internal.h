@@ -125,6 +125,13 @@ int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w,
 template <typename T>
 int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
 
+// Block-level trailing update hook: A22 -= L21 U12 for the step at column k
+// with panel width nb and trailing order m.
+typedef int (*TrailingUpdate)(nmx_ctx* ctx, void* user, int k, int nb, int m);
+template <typename T>
+int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base,
+                  TrailingUpdate update, void* user);
+
 template <typename T>
 int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);
 

@@ -25,7 +25,11 @@ import torch
 
 from . import _lib
 
-__all__ = ["LU", "NumaxError", "lu_factor", "solve", "cholesky", "cholesky_solve", "solve_triangular", "gemm"]
+__all__ = ["LU", "NumaxError", "lu_factor", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
+           "gemm", "PRECISIONS"]
+
+# trailing-update precisions of the mixed-precision solver (nmx_ir_cfg.precision)
+PRECISIONS = {"fp16": 0, "bf16": 1, "fp16s": 2, "fp32": 3}
 
 
 class NumaxError(ValueError):
@@ -229,3 +233,52 @@ def gemm(a, b, c=None, alpha: float = 1.0, beta: float = 0.0, transa: bool = Fal
     _lib.check(_fn("gemm", A.dtype)(ctx.handle, int(transa), int(transb), m, n, k, alpha, _ptr(A), A.shape[1],
                                     _ptr(B), B.shape[1], beta, _ptr(C), max(n, 1)), "gemm")
     return C
+
+
+def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block: int = 256,
+               seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial"):
+    """``x`` with ``A x = b`` to FP64 backward error from a low-precision
+    factorization (no reference counterpart; numax's ``solve`` is the FP64
+    path).  ``precision`` picks the trailing-update arithmetic: ``"fp16"``,
+    ``"bf16"``, ``"fp16s"`` (FP16 hi + lo split, ~FP32 accuracy) or ``"fp32"``.
+    Returns ``(x, report)``; ``report`` holds the refinement history and the
+    final normwise backward error."""
+    A = _device_tensor(a, torch.float64)
+    n = _square(A, "solve_mpir")
+    B = _device_tensor(b, torch.float64)
+    if B.dim() == 1:
+        cols = [B]
+    elif B.dim() == 2:
+        cols = [B[:, j].contiguous() for j in range(B.shape[1])]
+    else:
+        raise NumaxError("solve_mpir: b must be a vector or a matrix")
+    if B.shape[0] != n:
+        raise NumaxError(f"solve_mpir: b has {B.shape[0]} entries for an order-{n} matrix")
+    if precision not in PRECISIONS:
+        raise NumaxError(f"solve_mpir: precision must be one of {sorted(PRECISIONS)}")
+    lib = _lib.load()
+    cfg = _lib.IrCfg()
+    lib.nmx_ir_cfg_default(ctypes.byref(cfg))
+    cfg.precision = PRECISIONS[precision]
+    cfg.tol = tol
+    cfg.max_iter = max_iter
+    cfg.block = block
+    cfg.seed = seed
+    if pivot not in ("partial", "butterfly"):
+        raise NumaxError("solve_mpir: pivot must be 'partial' or 'butterfly'")
+    cfg.pivot = 1 if pivot == "partial" else 0
+    ctx = _ctx(A)
+    xs, reports = [], []
+    for col in cols:
+        x = torch.empty_like(col)
+        rep = _lib.IrReport()
+        st = lib.nmx_gesv_mpir(ctx.handle, n, _ptr(A), n, _ptr(col), _ptr(x), ctypes.byref(cfg), ctypes.byref(rep))
+        info = {"iterations": rep.iterations, "converged": bool(rep.converged), "backward_error": rep.backward_error,
+                "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms, "info": rep.info,
+                "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
+        if st != 0 and not (st == 5 and not raise_on_failure):
+            _lib.check(st, "solve_mpir")
+        xs.append(x)
+        reports.append(info)
+    x = xs[0] if B.dim() == 1 else torch.stack(xs, dim=1)
+    return x, (reports[0] if B.dim() == 1 else reports)

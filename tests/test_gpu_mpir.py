@@ -1,0 +1,89 @@
+"""GPU: mixed-precision LU with iterative refinement (no reference
+counterpart).  Pinned by the FP64 oracle's backward error: every result must
+reach the target normwise backward error ||b - A x|| / (||A|| ||x||) <= 1e-15
+and agree with the oracle's FP64 solution to what the condition number
+allows."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+from numax_amd import linalg as nl
+from numax_amd import random as R
+
+pytestmark = pytest.mark.gpu
+
+
+def haar(seed, n):
+    q, r = sl.qr(R.normal(seed, (n, n)))
+    return q * np.sign(np.diag(r))
+
+
+def svd_matrix(n, kappa, seed=3):
+    u, v = haar(seed, n), haar(seed + 100, n)
+    sigma = kappa ** (-np.arange(n) / (n - 1))
+    return (u * sigma) @ v.T
+
+
+def fp64_eta(a, b):
+    lu, piv, _, _ = oracle.getrf(a)
+    xo = oracle.getrs(lu, piv, b)
+    return oracle.backward_error(a, xo, b), xo
+
+
+@pytest.mark.parametrize("precision", ["fp16s", "fp16", "bf16", "fp32"])
+def test_mpir_reaches_fp64_backward_error(precision):
+    n = 1024
+    a = svd_matrix(n, 1e2)
+    b = R.normal(4, (n,))
+    x, rep = nl.solve_mpir(a, b, precision=precision, max_iter=50)
+    x = x.cpu().numpy()
+    eta = oracle.backward_error(a, x, b)
+    eta_o, xo = fp64_eta(a, b)
+    assert rep["converged"] and eta <= max(1e-15, eta_o), (rep, eta, eta_o)
+    assert np.max(np.abs(x - xo)) / np.max(np.abs(xo)) < 1e2 * 1e-14
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+def test_mpir_condition_numbers(kappa):
+    n = 2048
+    a = svd_matrix(n, kappa, seed=7)
+    b = R.normal(8, (n,))
+    x, rep = nl.solve_mpir(a, b, precision="fp16s")
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    eta_o, _ = fp64_eta(a, b)
+    assert rep["converged"] and eta <= max(1e-15, eta_o), (kappa, rep, eta, eta_o)
+
+
+@pytest.mark.parametrize("n", [5, 64, 257, 1001, 1030])
+def test_mpir_odd_sizes(n):
+    """n not a multiple of 4 (butterfly tail), smaller than a block, ragged
+    last block."""
+    a = R.normal(20 + n, (n, n)) + n ** 0.5 * np.eye(n)
+    b = R.normal(30 + n, (n,))
+    x, rep = nl.solve_mpir(a, b, precision="fp16s")
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    assert rep["converged"] and eta <= 1e-15, (n, rep, eta)
+
+
+@pytest.mark.parametrize("precision", ["fp16s", "fp32"])
+def test_mpir_butterfly_mode_small(precision):
+    """The opt-in unpivoted mode (random butterflies) on a diagonally strong
+    matrix, where no-pivot elimination is stable."""
+    n = 700
+    a = R.normal(40, (n, n)) + 4 * n ** 0.5 * np.eye(n)
+    b = R.normal(41, (n,))
+    x, rep = nl.solve_mpir(a, b, precision=precision, pivot="butterfly")
+    assert rep["converged"] and oracle.backward_error(a, x.cpu().numpy(), b) <= 1e-15
+
+
+def test_mpir_matrix_rhs_and_errors():
+    n = 300
+    a = svd_matrix(n, 1e3, seed=11)
+    bm = R.normal(12, (n, 3))
+    x, reps = nl.solve_mpir(a, bm)
+    assert x.shape == (n, 3) and all(r["converged"] for r in reps)
+    with pytest.raises(nl.NumaxError, match="entries"):
+        nl.solve_mpir(a, np.ones(n - 1))
+    with pytest.raises(nl.NumaxError, match="precision"):
+        nl.solve_mpir(a, bm[:, 0], precision="fp8")
