@@ -29,6 +29,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"hbm_GBs": 8000.0, "fp64_TFs": 78.6, "fp32_TFs": 157.3, "fp16_TFs": 2500.0, "bf16_TFs": 2500.0}
+# what bounds each kernel class of the library's profiler
+ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("hbm", None),
+        "laswp": ("hbm", None), "trsm_base": ("hbm", None), "residual": ("hbm", None), "mp_trsv": ("hbm", None),
+        "mp_panel": ("hbm", None), "potrf_diag": ("hbm", None), "other": ("hbm", None)}
+
+
+def profile_step(ctx, step) -> dict:
+    """Per-kernel-class HIP-event times of one step on the library's stream."""
+    ctx.set_profiling(True)
+    step()
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    log("profile: " + json.dumps({k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in prof.items()}))
+    return prof
 
 
 def log(msg: str) -> None:
@@ -95,7 +109,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--kappa", type=float, default=1e2)
+    ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--precision", default="fp16s", choices=["fp16", "bf16", "fp16s", "fp32"])
+    ap.add_argument("--tol", type=float, default=1e-15)
+    ap.add_argument("--all-kappa", action="store_true", help="also time kappa = 1e2, 1e4, 1e6")
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=4096)
@@ -154,20 +171,64 @@ def main() -> int:
     ctx.check_device()
     eta_plain = backward_error(a, x, b)
     log(f"plain fp64 backward error {eta_plain:.3e}")
+    prof_plain = profile_step(ctx, plain_step)
 
-    # one instrumented step: per-kernel-class event times on the library stream
-    ctx.set_profiling(True)
-    plain_step()
-    prof = ctx.profile()
-    ctx.set_profiling(False)
-    g = prof.get("gemm", {"ms": float("nan"), "flops": 0.0, "launches": 0})
-    gemm_tfs = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] else None
-    log("profile: " + json.dumps({k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in prof.items()}))
+    # ---- mixed-precision LU-IR on the same matrix, then at the other kappas
+    cfg = _lib.IrCfg()
+    lib.nmx_ir_cfg_default(ctypes.byref(cfg))
+    cfg.precision = nl.PRECISIONS[args.precision]
+    cfg.block = args.block
+    cfg.tol = args.tol
+    rep = _lib.IrReport()
+    xm = torch.empty(n, dtype=torch.float64, device=dev)
+    bvec = b if nrhs == 1 else b[:, 0].contiguous()
 
-    roofline = {"kernel": "gemm_kernel<double> (FP64 MFMA 16x16x4 trailing update)", "bound": "mfma",
-                "achieved": gemm_tfs, "peak": PEAK["fp64_TFs"], "unit": "TFLOP/s",
-                "frac": (gemm_tfs / PEAK["fp64_TFs"]) if gemm_tfs else None, "traffic": None,
-                "launches_per_step": g["launches"], "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
+    def mp_step():
+        st = lib.nmx_gesv_mpir(ctx.handle, n, P(a), n, P(bvec), P(xm), ctypes.byref(cfg), ctypes.byref(rep))
+        if st not in (0, 5):
+            _lib.check(st, "nmx_gesv_mpir")
+
+    results = {}
+    kappas = [args.kappa] + [k for k in (1e2, 1e4, 1e6) if k != args.kappa and args.all_kappa]
+    for kappa in kappas:
+        if kappa != args.kappa:
+            del a
+            torch.cuda.empty_cache()
+            a = c3_matrix(n, kappa, torch)
+        dt_mp = timed(mp_step, args.steps, args.warmup, f"mp-ir {args.precision} kappa={kappa:g}")
+        eta_mp = backward_error(a, xm, bvec)
+        results[f"{kappa:g}"] = {"ms_per_step": dt_mp * 1e3, "solves_per_s": 1.0 / dt_mp,
+                                 "effective_tflops": (2 * n ** 3 / 3 + 2 * n ** 2) / dt_mp / 1e12,
+                                 "iterations": rep.iterations, "converged": bool(rep.converged),
+                                 "backward_error": eta_mp, "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms,
+                                 "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
+        log(f"mp-ir kappa={kappa:g}: {json.dumps(results[f'{kappa:g}'])}")
+    head = results[f"{args.kappa:g}"]
+    # per-kernel profile of one MP solve at the headline kappa's matrix
+    if len(kappas) > 1:
+        a = c3_matrix(n, args.kappa, torch)
+    prof_mp = profile_step(ctx, mp_step)
+
+    def roofline_of(prof, label):
+        name, rec = max(prof.items(), key=lambda kv: kv[1]["ms"])
+        bound = ROOF.get(name, ("hbm", None))
+        sec = rec["ms"] * 1e-3
+        if bound[0] == "mfma":
+            ach = rec["flops"] / sec / 1e12
+            peak = PEAK[bound[1]]
+            unit = "TFLOP/s"
+        else:
+            ach = rec["bytes"] / sec / 1e9
+            peak = PEAK["hbm_GBs"]
+            unit = "GB/s"
+        return {"kernel": name, "path": label, "bound": bound[0], "achieved": ach, "peak": peak, "unit": unit,
+                "frac": ach / peak, "traffic": None, "launches_per_step": rec["launches"],
+                "avg_launch_ms": rec["ms"] / max(rec["launches"], 1), "share_of_step": None}
+
+    roof = roofline_of(prof_mp, "mp-ir")
+    roof["share_of_step"] = prof_mp[roof["kernel"]]["ms"] / head["ms_per_step"]
+    roof_plain = roofline_of(prof_plain, "plain-fp64")
+    roof_plain["share_of_step"] = prof_plain[roof_plain["kernel"]]["ms"] / (dt_plain * 1e3)
 
     cpu = None
     if not args.no_cpu:
@@ -176,7 +237,7 @@ def main() -> int:
         log(f"cpu baseline: oracle LU n={args.cpu_sample} on {threads} threads")
         cpu = cpu_baseline(args.cpu_sample, threads)
 
-    value = nrhs / dt_plain
+    value = 1.0 / (head["ms_per_step"] * 1e-3)
     out = {
         "metric": "solves/s + TFLOP/s at FP64 backward error: MP-IR vs plain FP64 LU, n=16384",
         "value": value,
@@ -184,20 +245,27 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt_plain * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": f"mixed: {args.precision} MFMA update / f32 panel / f64 residual",
         "data": "synthetic",
-        "config": {"workload": f"plain FP64 blocked LU with partial pivoting, factor + solve, n={n}, "
-                               f"nrhs={nrhs}, A=U diag(sigma) V^T (Haar U,V), kappa={args.kappa:g}, block "
-                               f"{args.block}", "n": n, "nrhs": nrhs, "kappa": args.kappa, "block": args.block},
-        "tflops": flops / dt_plain / 1e12,
-        "backward_error": eta_plain,
-        "roofline": roofline,
+        "config": {"workload": f"mixed-precision LU-IR solve, n={n}, 1 RHS, A = U diag(sigma) V^T (Haar U, V), "
+                               f"kappa={args.kappa:g}, partial pivoting, {args.precision} trailing update, "
+                               f"block {args.block}, refined to FP64 backward error <= {args.tol:g}",
+                   "n": n, "nrhs": 1, "kappa": args.kappa, "precision": args.precision, "block": args.block},
+        "tflops": head["effective_tflops"],
+        "backward_error": head["backward_error"],
+        "ir_iterations": head["iterations"],
+        "speedup_vs_plain_fp64": dt_plain * 1e3 / head["ms_per_step"],
+        "plain_fp64": {"ms_per_step": dt_plain * 1e3, "solves_per_s": nrhs / dt_plain,
+                       "tflops": flops / dt_plain / 1e12, "backward_error": eta_plain, "nrhs": nrhs,
+                       "roofline": roof_plain, "kernel_profile": prof_plain},
+        "mp_by_kappa": results,
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "kernel_profile": prof,
+        "kernel_profile": prof_mp,
     }
     print(json.dumps(out), flush=True)
     return 0

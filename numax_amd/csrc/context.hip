@@ -44,6 +44,7 @@ NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
   }
   if (hipMalloc(&c->err, 256) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
   if (hipMalloc(&c->mailbox, kMailboxBytes) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
+  if (hipMalloc(&c->tri_inv, 64 * 64 * sizeof(double)) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
   c->mailbox_bytes = kMailboxBytes;
   size_t words = kMailboxBytes / 4;
   fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
@@ -64,6 +65,8 @@ NMX_API int nmx_ctx_destroy(nmx_ctx* c) {
   if (c->ws) hipFree(c->ws);
   if (c->mailbox) hipFree(c->mailbox);
   if (c->err) hipFree(c->err);
+  if (c->tri_inv) hipFree(c->tri_inv);
+  if (c->mp_cache && c->mp_cache_free) c->mp_cache_free(c->mp_cache);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;
   return NMX_OK;
@@ -126,7 +129,7 @@ int nmx_mailbox(nmx_ctx* c, void** out) {
 
 int nmx_epoch_reserve(nmx_ctx* c, uint32_t columns, uint32_t* base) {
   if ((uint64_t)c->epoch + columns + 4 >= 0x7fffffffu) {
-    size_t words = kMailboxFlagsBytes / 4;
+    size_t words = kMailboxBytes / 4;
     fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
     NMX_HIP(c, hipGetLastError());
     c->epoch = 0;

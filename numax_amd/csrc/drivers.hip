@@ -19,10 +19,7 @@ namespace {
 // After every leaf its interchanges are replayed over all other columns.
 template <typename T>
 int panel_rec(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, int base) {
-  if (nb <= base) {
-    NMX_TRY(getf2_leaf_launch<T>(ctx, n, A, lda, k, k, nb, ipiv, info));
-    return laswp_launch<T>(ctx, A, lda, 0, n, k, k + nb, ipiv, k, k + nb);
-  }
+  if (nb <= base) return getf2_leaf_launch<T>(ctx, n, A, lda, k, k, nb, ipiv, info, 1);
   int half = nb / 2;
   if (half > base) half = (half / base) * base;
   NMX_TRY(panel_rec<T>(ctx, n, A, lda, ipiv, info, k, half, base));

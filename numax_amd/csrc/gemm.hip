@@ -90,37 +90,47 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha,
     for (int j = 0; j < 4; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
   T ra[8], rb[8];
+  // Slab loads through buffer descriptors: an element outside the operand
+  // gets an offset past the descriptor's range and reads as 0, so edge and
+  // interior tiles run the same branch-free code with all loads in flight.
+  constexpr uint32_t E = sizeof(T);
   auto load_tiles = [&](int k0) {
-    // op(A) tile BM x BK
     if (!TA) {
-      int r = t >> 1, kh = (t & 1) * 8, gm = m0 + r;
+      const int r = t >> 1, kh = (t & 1) * 8, gm = m0 + r;
+      const __amdgpu_buffer_rsrc_t ra_ = buf_rsrc(A + (int64_t)m0 * lda);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        int gk = k0 + kh + i;
-        ra[i] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : T(0);
+        const int gk = k0 + kh + i;
+        const uint32_t off = ((gm < M) & (gk < K)) ? ((uint32_t)r * (uint32_t)lda + (uint32_t)gk) * E : kBufOob;
+        ra[i] = buf_load<T>(ra_, off);
       }
     } else {
-      int kk = t >> 4, mc = (t & 15) * 8, gk = k0 + kk;
+      const int kk = t >> 4, mc = (t & 15) * 8, gk = k0 + kk;
+      const __amdgpu_buffer_rsrc_t ra_ = buf_rsrc(A + (int64_t)k0 * lda + m0);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        int gm = m0 + mc + i;
-        ra[i] = (gm < M && gk < K) ? A[(int64_t)gk * lda + gm] : T(0);
+        const int gm = m0 + mc + i;
+        const uint32_t off = ((gm < M) & (gk < K)) ? ((uint32_t)kk * (uint32_t)lda + (uint32_t)(mc + i)) * E : kBufOob;
+        ra[i] = buf_load<T>(ra_, off);
       }
     }
-    // op(B) tile BK x BN
     if (!TB) {
-      int kk = t >> 4, nc = (t & 15) * 8, gk = k0 + kk;
+      const int kk = t >> 4, nc = (t & 15) * 8, gk = k0 + kk;
+      const __amdgpu_buffer_rsrc_t rb_ = buf_rsrc(B + (int64_t)k0 * ldb + n0);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        int gn = n0 + nc + i;
-        rb[i] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : T(0);
+        const int gn = n0 + nc + i;
+        const uint32_t off = ((gn < N) & (gk < K)) ? ((uint32_t)kk * (uint32_t)ldb + (uint32_t)(nc + i)) * E : kBufOob;
+        rb[i] = buf_load<T>(rb_, off);
       }
     } else {
-      int r = t >> 1, kh = (t & 1) * 8, gn = n0 + r;
+      const int r = t >> 1, kh = (t & 1) * 8, gn = n0 + r;
+      const __amdgpu_buffer_rsrc_t rb_ = buf_rsrc(B + (int64_t)n0 * ldb);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        int gk = k0 + kh + i;
-        rb[i] = (gn < N && gk < K) ? B[(int64_t)gn * ldb + gk] : T(0);
+        const int gk = k0 + kh + i;
+        const uint32_t off = ((gn < N) & (gk < K)) ? ((uint32_t)r * (uint32_t)ldb + (uint32_t)gk) * E : kBufOob;
+        rb[i] = buf_load<T>(rb_, off);
       }
     }
   };
@@ -146,26 +156,34 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha,
   };
 
   const int ksteps = (K + BK - 1) / BK;
-  if (ksteps > 0) {
-    load_tiles(0);
-    store_tiles(0);
-    __syncthreads();
-  }
+  load_tiles(0);
+  store_tiles(0);
+  __syncthreads();
   for (int s = 0; s < ksteps; s++) {
     const int buf = s & 1;
     if (s + 1 < ksteps) load_tiles((s + 1) * BK);
+    // fragments for k-step kq+1 are read while kq's MFMAs issue
+    T af[2][4], bf[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      af[0][i] = As[buf][lane >> 4][wm * 64 + i * 16 + (lane & 15)];
+      bf[0][i] = Bs[buf][lane >> 4][wn * 64 + i * 16 + (lane & 15)];
+    }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; kq++) {
-      const int kr = kq * 4 + (lane >> 4);
-      T af[4], bf[4];
+      const int cur = kq & 1;
+      if (kq + 1 < BK / 4) {
+        const int kr = (kq + 1) * 4 + (lane >> 4);
 #pragma unroll
-      for (int i = 0; i < 4; i++) af[i] = As[buf][kr][wm * 64 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 4; j++) bf[j] = Bs[buf][kr][wn * 64 + j * 16 + (lane & 15)];
+        for (int i = 0; i < 4; i++) {
+          af[cur ^ 1][i] = As[buf][kr][wm * 64 + i * 16 + (lane & 15)];
+          bf[cur ^ 1][i] = Bs[buf][kr][wn * 64 + i * 16 + (lane & 15)];
+        }
+      }
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; j++) acc[i][j] = Mfma<T>::mma(af[cur][i], bf[cur][j], acc[i][j]);
     }
     if (s + 1 < ksteps) {
       store_tiles(buf ^ 1);
@@ -173,7 +191,25 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha,
     }
   }
 
-  // ---- epilogue straight into the strided C ----
+  // ---- epilogue straight into the strided C: every C load is issued
+  // before any is used (buffer loads, out-of-range lanes read 0)
+  T cv[4][4][4];
+  const bool use_c = beta != T(0);
+  if (use_c) {
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int lr = wm * 64 + i * 16 + Mfma<T>::row(lane, e);
+          const int lc = wn * 64 + j * 16 + (lane & 15);
+          const uint32_t off =
+              ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * E : kBufOob;
+          cv[i][j][e] = buf_load<T>(rc, off);
+        }
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -182,12 +218,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha,
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e);
-        if (row < M && col < N) {
-          T* cp = C + (int64_t)row * ldc + col;
-          T v = alpha * acc[i][j][e];
-          if (beta != T(0)) v += beta * *cp;
-          *cp = v;
-        }
+        T v = alpha * acc[i][j][e];
+        if (use_c) v += beta * cv[i][j][e];
+        if ((row < M) & (col < N)) C[(int64_t)row * ldc + col] = v;
       }
     }
 }

@@ -334,8 +334,8 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
   typedef typename Lo<BF>::t lt;
   typedef lt v8 __attribute__((ext_vector_type(8)));
   constexpr int NOP = SPLIT ? 2 : 1;
-  __shared__ __attribute__((aligned(16))) lt As[NOP][MT][KT + KP];
-  __shared__ __attribute__((aligned(16))) lt Bs[NOP][MT][KT + KP];
+  __shared__ __attribute__((aligned(16))) lt As[2][NOP][MT][KT + KP];
+  __shared__ __attribute__((aligned(16))) lt Bs[2][NOP][MT][KT + KP];
 
   const int total = tilesM * tilesN;
   const int b = blockIdx.x;
@@ -354,16 +354,21 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
 
   // prefetch this lane's C values (the MFMA output layout)
   float cpre[2][2][16];
+  {
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+      for (int j = 0; j < 2; j++)
 #pragma unroll
-      for (int e = 0; e < 16; e++) {
-        const int row = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-        cpre[i][j][e] = (row < M && col < N) ? C[(int64_t)row * ldc + col] : 0.f;
-      }
+        for (int e = 0; e < 16; e++) {
+          const int lr = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+          const int lc = wn * 64 + j * 32 + (lane & 31);
+          const uint32_t off =
+              ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * 4u : kBufOob;
+          cpre[i][j][e] = buf_load<float>(rc, off);
+        }
+  }
 
   f16v acc[2][2], accx[2][2];
 #pragma unroll
@@ -374,46 +379,60 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
       accx[i][j] = f16v{};
     }
 
-  // global -> LDS: each operand slab is 128 rows x 32 halves = 512 chunks of 8
-  for (int k0 = 0; k0 < K; k0 += KT) {
+  // global -> registers -> LDS, double-buffered: slab s+1 is in flight while
+  // slab s feeds the MFMAs.  Each slab is 128 rows x 32 K values per operand
+  // = 512 chunks of 8, two per thread.
+  const __amdgpu_buffer_rsrc_t rAh = buf_rsrc(Ah + (int64_t)m0 * K);
+  const __amdgpu_buffer_rsrc_t rBh = buf_rsrc(Bh + (int64_t)n0 * K);
+  const __amdgpu_buffer_rsrc_t rAl = buf_rsrc((SPLIT ? Al : Ah) + (int64_t)m0 * K);
+  const __amdgpu_buffer_rsrc_t rBl = buf_rsrc((SPLIT ? Bl : Bh) + (int64_t)n0 * K);
+  v8 pa[NOP][2], pb[NOP][2];
+  auto load_slab = [&](int k0) {
 #pragma unroll
-    for (int op = 0; op < NOP; op++) {
-      const lt* srcA = op == 0 ? Ah : Al;
-      const lt* srcB = op == 0 ? Bh : Bl;
+    for (int c = 0; c < 2; c++) {
+      const int chunk = t + c * 256;
+      const int r = chunk >> 2, kk = (chunk & 3) * 8;
+      const int gk = k0 + kk;
+      const uint32_t off = ((uint32_t)r * (uint32_t)K + (uint32_t)gk) * 2u;
+      const uint32_t offA = ((m0 + r < M) & (gk < K)) ? off : kBufOob;
+      const uint32_t offB = ((n0 + r < N) & (gk < K)) ? off : kBufOob;
+      pa[0][c] = __builtin_bit_cast(v8, __builtin_amdgcn_raw_buffer_load_b128(rAh, offA, 0, 0));
+      pb[0][c] = __builtin_bit_cast(v8, __builtin_amdgcn_raw_buffer_load_b128(rBh, offB, 0, 0));
+      if (SPLIT) {
+        pa[NOP - 1][c] = __builtin_bit_cast(v8, __builtin_amdgcn_raw_buffer_load_b128(rAl, offA, 0, 0));
+        pb[NOP - 1][c] = __builtin_bit_cast(v8, __builtin_amdgcn_raw_buffer_load_b128(rBl, offB, 0, 0));
+      }
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int op = 0; op < NOP; op++)
 #pragma unroll
       for (int c = 0; c < 2; c++) {
         const int chunk = t + c * 256;
         const int r = chunk >> 2, kk = (chunk & 3) * 8;
-        v8 va, vb;
-        const int gr = m0 + r, gc = n0 + r, gk = k0 + kk;
-        if (gr < M && gk + 8 <= K) {
-          va = *(const v8*)(srcA + (int64_t)gr * K + gk);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; q++) va[q] = (gr < M && gk + q < K) ? srcA[(int64_t)gr * K + gk + q] : (lt)0.0f;
-        }
-        if (gc < N && gk + 8 <= K) {
-          vb = *(const v8*)(srcB + (int64_t)gc * K + gk);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; q++) vb[q] = (gc < N && gk + q < K) ? srcB[(int64_t)gc * K + gk + q] : (lt)0.0f;
-        }
-        *(v8*)&As[op][r][kk] = va;
-        *(v8*)&Bs[op][r][kk] = vb;
+        *(v8*)&As[buf][op][r][kk] = pa[op][c];
+        *(v8*)&Bs[buf][op][r][kk] = pb[op][c];
       }
-    }
-    __syncthreads();
+  };
+  const int nslab = (K + KT - 1) / KT;
+  load_slab(0);
+  store_slab(0);
+  __syncthreads();
+  for (int sl = 0; sl < nslab; sl++) {
+    const int buf = sl & 1;
+    if (sl + 1 < nslab) load_slab((sl + 1) * KT);
 #pragma unroll
     for (int ks = 0; ks < KT; ks += 16) {
       const int ko = ks + 8 * (lane >> 5);
       v8 a0[2], b0[2], a1[2], b1[2];
 #pragma unroll
       for (int i = 0; i < 2; i++) {
-        a0[i] = *(const v8*)&As[0][wm * 64 + i * 32 + (lane & 31)][ko];
-        b0[i] = *(const v8*)&Bs[0][wn * 64 + i * 32 + (lane & 31)][ko];
+        a0[i] = *(const v8*)&As[buf][0][wm * 64 + i * 32 + (lane & 31)][ko];
+        b0[i] = *(const v8*)&Bs[buf][0][wn * 64 + i * 32 + (lane & 31)][ko];
         if (SPLIT) {
-          a1[i] = *(const v8*)&As[NOP - 1][wm * 64 + i * 32 + (lane & 31)][ko];
-          b1[i] = *(const v8*)&Bs[NOP - 1][wn * 64 + i * 32 + (lane & 31)][ko];
+          a1[i] = *(const v8*)&As[buf][NOP - 1][wm * 64 + i * 32 + (lane & 31)][ko];
+          b1[i] = *(const v8*)&Bs[buf][NOP - 1][wn * 64 + i * 32 + (lane & 31)][ko];
         }
       }
 #pragma unroll
@@ -427,7 +446,10 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
           }
         }
     }
-    __syncthreads();
+    if (sl + 1 < nslab) {
+      store_slab(buf ^ 1);
+      __syncthreads();
+    }
   }
 
   const float inv = 1.0f / (pow2_scale(*maxA) * pow2_scale(*maxB));
@@ -439,11 +461,10 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
       for (int e = 0; e < 16; e++) {
         const int row = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
         const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-        if (row < M && col < N) {
-          float prod = acc[i][j][e];
-          if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
-          C[(int64_t)row * ldc + col] = cpre[i][j][e] - prod * inv;
-        }
+        float prod = acc[i][j][e];
+        if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
+        const float out = cpre[i][j][e] - prod * inv;
+        if ((row < M) & (col < N)) C[(int64_t)row * ldc + col] = out;
       }
 }
 
@@ -501,24 +522,32 @@ constexpr long kTrsvSpin = 1L << 23;
 // per-lane partial sums are reduced across the wavefront only at the end.
 __device__ __forceinline__ void tile_gemv_accum(float s[32], const float* __restrict__ tile, int64_t ld, int rows,
                                                 int cols, const float* v, int wave, int lane) {
+  // all 64 loads of a lane in flight at once: buffer loads, out-of-range
+  // elements read 0 (v holds 0 past `cols`)
+  const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tile);
+  const float v0 = v[lane], v1 = v[lane + 64];
+  float a0[32], a1[32];
 #pragma unroll
   for (int q = 0; q < 32; q++) {
     const int r = wave + 4 * q;
-    if (r < rows) {
-      const float* row = tile + (int64_t)r * ld;
-      float a = 0.f;
-      if (lane < cols) a += row[lane] * v[lane];
-      if (lane + 64 < cols) a += row[lane + 64] * v[lane + 64];
-      s[q] += a;
-    }
+    const uint32_t base = (uint32_t)r * (uint32_t)ld;
+    a0[q] = buf_load<float>(rt, ((r < rows) & (lane < cols)) ? (base + lane) * 4u : kBufOob);
+    a1[q] = buf_load<float>(rt, ((r < rows) & (lane + 64 < cols)) ? (base + lane + 64) * 4u : kBufOob);
   }
+#pragma unroll
+  for (int q = 0; q < 32; q++) s[q] += a0[q] * v0 + a1[q] * v1;
 }
 
 template <int UPPER>
 __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* __restrict__ F, int ldf,
                                                            const float* __restrict__ inv, float* __restrict__ x,
-                                                           uint32_t* __restrict__ flags, uint32_t epoch,
-                                                           int* __restrict__ err) {
+                                                           uint64_t* __restrict__ xpub, uint32_t epoch,
+                                                           int* __restrict__ err, uint64_t* __restrict__ stamps) {
+  // Blocks are solved in dispatch order (lower: first block first; upper:
+  // last block first), and a block only waits for blocks dispatched before
+  // it, so the chain needs no co-residency guarantee.  Each solved entry is
+  // published as one 64-bit word (value bits, epoch): a reader validates
+  // every word it uses on its own, with no fence.
   __shared__ float xs[TB];
   __shared__ float rhs[TB];
   __shared__ int aborted;
@@ -535,15 +564,28 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* _
   for (int st = 0; st < steps; st++) {
     const int bj = UPPER ? nblk - 1 - st : st;
     const int j0 = bj * TB, jb = min(TB, n - j0);
+    // one poller per workgroup (with backoff) keeps the spinning off the
+    // producer's cache lines; then every thread reads and validates its word
     if (t == 0) {
       long spins = 0;
-      while ((int)(__hip_atomic_load(&flags[bj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
+      while ((uint32_t)__hip_atomic_load(&xpub[j0 + jb - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
         if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
+        __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    if (t < jb) xs[t] = x[j0 + t];
+    if (t >= jb && t < TB) xs[t] = 0.f;
+    if (t < jb) {
+      uint64_t wv;
+      long spins = 0;
+      for (;;) {
+        wv = __hip_atomic_load(&xpub[j0 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)wv == epoch) break;
+        if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      xs[t] = __uint_as_float((uint32_t)(wv >> 32));
+    }
     __syncthreads();
     tile_gemv_accum(s, F + (int64_t)k0 * ldf + j0, ldf, nb, jb, xs, wave, lane);
     __syncthreads();
@@ -557,7 +599,7 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* _
     if (lane == 0 && r < TB) rhs[r] = (r < nb) ? x[k0 + r] - v : 0.f;
   }
   __syncthreads();
-  // x_i = inv_ii rhs
+  // x_i = inv_ii rhs, stored plain and published tagged
 #pragma unroll
   for (int q = 0; q < 32; q++) s[q] = 0.f;
   tile_gemv_accum(s, inv + (int64_t)bi * TB * TB, TB, nb, nb, rhs, wave, lane);
@@ -566,13 +608,14 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* _
     float v = s[q];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     const int r = wave + 4 * q;
-    if (lane == 0 && r < nb) x[k0 + r] = v;
+    if (lane == 0 && r < nb) {
+      x[k0 + r] = v;
+      __hip_atomic_store(&xpub[k0 + r], ((uint64_t)__float_as_uint(v) << 32) | epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
-  __syncthreads();
-  if (t == 0) {
-    __hip_atomic_store(&flags[bi], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
-  }
+  if (t == 0 && aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
+  if (stamps && t == 0) stamps[bi] = __builtin_amdgcn_s_memrealtime();  // 100 MHz clock
 }
 
 // ---------------------------------------------------------------------------
@@ -587,12 +630,24 @@ __global__ void __launch_bounds__(256) residual_kernel(int n, const double* __re
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n) return;
-  const double* a = A + (int64_t)row * lda;
+  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)row * lda);
+  const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x);
   double s = 0.0, sa = 0.0;
-  for (int c = lane; c < n; c += 64) {
-    const double av = a[c];
-    s += av * x[c];
-    if (rowabs) sa += fabs(av);
+  // 8 column chunks of 64 in flight per step; past-the-end reads are 0
+  for (int c0 = 0; c0 < n; c0 += 512) {
+    double av[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = c0 + u * 64 + lane;
+      const uint32_t off = c < n ? (uint32_t)c * 8u : kBufOob;
+      av[u] = buf_load<double>(ra, off);
+      xv[u] = buf_load<double>(rx, off);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      s += av[u] * xv[u];
+      sa += fabs(av[u]);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_xor(s, off);
@@ -645,7 +700,7 @@ struct MpFactor {
   int n = 0, block = 256, prec = 2;  // prec: 0 fp16, 1 bf16, 2 fp16 split, 3 fp32
   int pivot = 1;                     // 1: partial pivoting (GEPP); 0: butterflies, no pivoting
   int32_t* ipiv = nullptr;
-  float* F = nullptr;                // n x n FP32 factors of W^T A V
+  float* F = nullptr;                // n x n FP32 factors (of W^T A V without pivoting)
   float* dw = nullptr;               // 2n butterfly diagonals for W, then V
   float* dv = nullptr;
   float* invL = nullptr;             // nblk x 128 x 128
@@ -653,41 +708,96 @@ struct MpFactor {
   int32_t* info = nullptr;
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
   unsigned* maxbits = nullptr;       // [2] per step
-  uint32_t* flags = nullptr;         // TRSV flags (nblk)
+  uint64_t* xpub = nullptr;          // TRSV published solution words (n)
   uint32_t epoch = 0;
   bool ok = false;
 };
+
+// Device buffers of the factorization, cached in the context and grown on
+// demand so repeated solves do not allocate.
+struct MpBuffers {
+  size_t F_elems = 0, d_elems = 0, inv_elems = 0, piv_elems = 0, lo_bytes = 0, pub_elems = 0;
+  float* F = nullptr;
+  float* d = nullptr;
+  float* inv = nullptr;
+  int32_t* ipiv = nullptr;
+  int32_t* info = nullptr;
+  void* lo = nullptr;
+  unsigned* maxbits = nullptr;
+  uint64_t* xpub = nullptr;
+  uint32_t epoch = 0;
+};
+
+static void mp_buffers_free(void* p) {
+  MpBuffers* b = (MpBuffers*)p;
+  if (!b) return;
+  hipFree(b->F);
+  hipFree(b->d);
+  hipFree(b->inv);
+  hipFree(b->ipiv);
+  hipFree(b->info);
+  hipFree(b->lo);
+  hipFree(b->maxbits);
+  hipFree(b->xpub);
+  delete b;
+}
+
+template <typename P>
+static int grow(nmx_ctx* ctx, P*& ptr, size_t& have, size_t want, size_t elem) {
+  if (want <= have && ptr) return NMX_OK;
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  have = 0;
+  if (hipMalloc((void**)&ptr, want * elem + 256) != hipSuccess) return NMX_ERR_ALLOC;
+  have = want;
+  return NMX_OK;
+}
 
 static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.n = n;
   f.block = block;
   f.prec = prec;
+  MpBuffers* b = (MpBuffers*)ctx->mp_cache;
+  if (!b) {
+    b = new MpBuffers();
+    ctx->mp_cache = b;
+    ctx->mp_cache_free = mp_buffers_free;
+  }
+  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const int nblk = ceil_div(n, TB);
-  size_t lo_elems = (size_t)n * block * 4;  // A_hi, A_lo, B_hi, B_lo (lo unused unless split)
-  NMX_HIP(ctx, hipMalloc(&f.F, sizeof(float) * (size_t)n * n));
-  NMX_HIP(ctx, hipMalloc(&f.dw, sizeof(float) * 4 * (size_t)n + 64));
-  f.dv = f.dw + 2 * (size_t)n;
-  NMX_HIP(ctx, hipMalloc(&f.invL, sizeof(float) * (size_t)nblk * TB * TB * 2));
-  f.invU = f.invL + (size_t)nblk * TB * TB;
-  NMX_HIP(ctx, hipMalloc(&f.info, 256));
-  NMX_HIP(ctx, hipMalloc(&f.ipiv, sizeof(int32_t) * (size_t)n + 256));
-  NMX_HIP(ctx, hipMalloc(&f.lo_buf, 2 * lo_elems + 256));
-  NMX_HIP(ctx, hipMalloc(&f.maxbits, 256));
-  NMX_HIP(ctx, hipMalloc(&f.flags, sizeof(uint32_t) * (size_t)nblk + 256));
-  hipMemsetAsync(f.flags, 0, sizeof(uint32_t) * (size_t)nblk, ctx->stream);
-  NMX_HIP(ctx, hipGetLastError());
+  size_t lo_bytes = (size_t)n * block * 4 * 2;  // A_hi, A_lo, B_hi, B_lo (lo unused unless split)
+  size_t unused = 0, unused2 = 0;
+  NMX_TRY(grow(ctx, b->F, b->F_elems, (size_t)n * n, sizeof(float)));
+  NMX_TRY(grow(ctx, b->d, b->d_elems, 4 * (size_t)n + 16, sizeof(float)));
+  NMX_TRY(grow(ctx, b->inv, b->inv_elems, (size_t)nblk * TB * TB * 2, sizeof(float)));
+  NMX_TRY(grow(ctx, b->ipiv, b->piv_elems, (size_t)n, sizeof(int32_t)));
+  NMX_TRY(grow(ctx, b->lo, b->lo_bytes, lo_bytes, 1));
+  if (!b->info) NMX_TRY(grow(ctx, b->info, unused, 64, sizeof(int32_t)));
+  if (!b->maxbits) NMX_TRY(grow(ctx, b->maxbits, unused2, 64, sizeof(unsigned)));
+  if ((size_t)n > b->pub_elems) {
+    NMX_TRY(grow(ctx, b->xpub, b->pub_elems, (size_t)n, sizeof(uint64_t)));
+    hipMemsetAsync(b->xpub, 0, sizeof(uint64_t) * (size_t)n, ctx->stream);
+    NMX_HIP(ctx, hipGetLastError());
+    b->epoch = 0;
+  }
+  f.F = b->F;
+  f.dw = b->d;
+  f.dv = b->d + 2 * (size_t)n;
+  f.invL = b->inv;
+  f.invU = b->inv + (size_t)nblk * TB * TB;
+  f.ipiv = b->ipiv;
+  f.info = b->info;
+  f.lo_buf = b->lo;
+  f.maxbits = b->maxbits;
+  f.xpub = b->xpub;
+  f.epoch = b->epoch;
   return NMX_OK;
 }
 
-static void mp_free(MpFactor& f) {
-  if (f.F) hipFree(f.F);
-  if (f.dw) hipFree(f.dw);
-  if (f.invL) hipFree(f.invL);
-  if (f.info) hipFree(f.info);
-  if (f.ipiv) hipFree(f.ipiv);
-  if (f.lo_buf) hipFree(f.lo_buf);
-  if (f.maxbits) hipFree(f.maxbits);
-  if (f.flags) hipFree(f.flags);
+// Hands the TRSV epoch back to the cache (flags keep their last values).
+static void mp_free(nmx_ctx* ctx, MpFactor& f) {
+  MpBuffers* b = (MpBuffers*)ctx->mp_cache;
+  if (b) b->epoch = f.epoch;
   f = MpFactor();
 }
 
@@ -829,16 +939,12 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   const int nblk = ceil_div(n, TB);
   for (int up = 0; up < 2; up++) {
     f.epoch += 1;
-    uint32_t ep = f.epoch;
-    const float* F = f.F;
-    const float* inv = up ? f.invU : f.invL;
-    int nn = n, ld = n;
-    uint32_t* flags = f.flags;
-    int* err = ctx->err;
-    void* args[] = {&nn, (void*)&F, &ld, (void*)&inv, &tmp, &flags, &ep, &err};
     ProfScope ps(ctx, K_MP_TRSV, 2.0 * (double)n * n / 2, 4.0 * (double)n * n / 2);
-    const void* fn = up ? (const void*)trsv_blocked_kernel<1> : (const void*)trsv_blocked_kernel<0>;
-    NMX_HIP(ctx, hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, ctx->stream));
+    if (up)
+      trsv_blocked_kernel<1><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invU, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+    else
+      trsv_blocked_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+    NMX_HIP(ctx, hipGetLastError());
   }
   if (f.pivot) {
     vec_f32_add_f64<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, tmp, x, accumulate);
@@ -895,7 +1001,7 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   void* wsp = nullptr;
   if (st == NMX_OK) st = nmx_workspace(ctx, sizeof(double) * (3 * (size_t)n + 8) + sizeof(float) * (size_t)n, &wsp);
   if (st != NMX_OK) {
-    mp_free(f);
+    mp_free(ctx, f);
     return st;
   }
   double* r = (double*)wsp;
@@ -947,7 +1053,7 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   R->factor_ms = fms;
   R->refine_ms = rms;
   R->info = hinfo;
-  mp_free(f);
+  mp_free(ctx, f);
   if (st != NMX_OK) return st;
   int dev = nmx_ctx_check(ctx);
   if (dev != NMX_OK) return dev;
@@ -987,6 +1093,6 @@ NMX_API int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, i
     if (ipiv) hipMemcpyAsync(ipiv, f.ipiv, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = NMX_ERR_HIP;
   }
-  mp_free(f);
+  mp_free(ctx, f);
   return st;
 }

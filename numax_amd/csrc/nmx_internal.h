@@ -69,6 +69,11 @@ struct nmx_ctx {
   uint32_t epoch = 0;  // last epoch handed to a kernel (monotonic)
   // device error word (sticky): NMX_ERR_SYNC_TIMEOUT from bounded waits
   int* err = nullptr;
+  // 64 x 64 scratch for the inverse of a triangular base block (FP64-sized)
+  void* tri_inv = nullptr;
+  // buffers of the mixed-precision factorization, kept across calls
+  void* mp_cache = nullptr;
+  void (*mp_cache_free)(void*) = nullptr;
   // event profiler (off unless nmx_ctx_set_profiling)
   bool profiling = false;
   std::vector<ProfRec> recs;
@@ -90,11 +95,9 @@ struct ProfScope {
 // when needed.
 int nmx_workspace(nmx_ctx* ctx, size_t bytes, void** out);
 
-// Mailbox sized for `parts` participants with `w` values of `elem` bytes.
+// Cross-workgroup mailbox of the cooperative panel (epoch-tagged 64-bit words).
 int nmx_mailbox(nmx_ctx* ctx, void** out);
-constexpr size_t kMailboxFlagsBytes = kMaxParts * 64;                  // one flag per 64 B line
-constexpr size_t kMailboxSlotBytes = 64 + kLeafW * sizeof(double);    // mag,row + a row of values
-constexpr size_t kMailboxBytes = kMailboxFlagsBytes + 2 * (kMaxParts + 1) * kMailboxSlotBytes;
+constexpr size_t kMailboxBytes = 64 * 1024;
 
 // Advance and return the epoch base for `columns` upcoming columns; resets
 // the flags (with a fill kernel) before the 32-bit counter could wrap.
@@ -120,7 +123,8 @@ int laswp_launch(nmx_ctx* ctx, T* A, int lda, int col_lo, int col_hi, int skip_l
                  int k1, int k2);
 
 template <typename T>
-int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info);
+int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info,
+                      int swap_others);
 
 template <typename T>
 int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
@@ -136,3 +140,21 @@ template <typename T>
 int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// ---- buffer loads: the hardware range check returns 0 past num_records, so
+// bounds-checked tile loads need no branches ---------------------------------
+constexpr uint32_t kBufOob = 0xFFFFFFF0u;  // an offset no descriptor covers
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}

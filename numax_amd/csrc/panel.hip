@@ -22,18 +22,36 @@ __device__ __forceinline__ bool better(T ma, int ra, T mb, int rb) {
   return ma > mb || (ma == mb && ra < rb);
 }
 
-// ---- mailbox layout --------------------------------------------------------
-struct Slot {
-  double mag;
-  int row;
-  int pad[13];
-  double vals[kLeafW];
+// ---- mailbox ----------------------------------------------------------------
+// Every word a participant publishes is a 64-bit atomic holding 32 bits of
+// payload and the 32-bit epoch of the column it belongs to, so a reader
+// validates each word on its own and needs no fence and no second trip.  A
+// double travels as two words.  Slots are double-buffered by epoch parity: a
+// participant can only publish column j+2 after every participant has
+// published j+1, which each does only after reading all of column j.
+template <typename T>
+struct MbLayout {
+  static constexpr int VW = sizeof(T) / 4;       // words per value
+  static constexpr int STRIDE = 1 + VW + kLeafW * VW;  // header, |pivot|, row values
+  static constexpr int PARITY = kMaxParts * STRIDE + kLeafW * VW;  // + the diagonal row
 };
-static_assert(sizeof(Slot) == kMailboxSlotBytes, "mailbox slot size");
+static_assert(2 * MbLayout<double>::PARITY * 8 <= kMailboxBytes, "mailbox too small");
 
-__device__ __forceinline__ uint32_t* mb_flag(unsigned char* mb, int p) { return (uint32_t*)(mb + (size_t)p * 64); }
-__device__ __forceinline__ Slot* mb_slot(unsigned char* mb, int parity, int s) {
-  return (Slot*)(mb + kMailboxFlagsBytes) + (size_t)parity * (kMaxParts + 1) + s;
+__device__ __forceinline__ uint64_t mb_pack(uint32_t payload, uint32_t epoch) {
+  return ((uint64_t)payload << 32) | epoch;
+}
+__device__ __forceinline__ void mb_put(uint64_t* w, uint32_t payload, uint32_t epoch) {
+  __hip_atomic_store(w, mb_pack(payload, epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void mb_put_value(uint64_t* w, T v, uint32_t epoch) {
+  if (sizeof(T) == 8) {
+    const uint64_t bits = __double_as_longlong((double)v);
+    mb_put(w, (uint32_t)bits, epoch);           // low half first
+    mb_put(w + 1, (uint32_t)(bits >> 32), epoch);
+  } else {
+    mb_put(w, __float_as_uint((float)v), epoch);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -44,24 +62,24 @@ __device__ __forceinline__ Slot* mb_slot(unsigned char* mb, int parity, int s) {
 // (ties to the lower row; the diagonal row counts as magnitude >= 0, so an
 // all-zero or all-NaN column keeps its diagonal), interchange, an exact-zero
 // pivot divided as 1e-30 with `info` recording the first, true division for
-// the multipliers and the rank-1 update of the panel's own columns.  Only the
-// leaf's columns are interchanged here; the driver replays the leaf's pivots
-// over every other column afterwards (laswp), which leaves the same matrix as
-// numax's full-width swaps.
+// the multipliers and the rank-1 update of the panel's own columns.  The
+// leaf's interchanges are then replayed over every other column of the
+// matrix right after (laswp_gather_kernel), which leaves the matrix numax's
+// full-width swaps leave (panel.mojo:404-411).
 //
-// Workgroup p of `parts` owns rows r0 + p*R ... (R = RPT*1024), thread t
-// holds row r0 + p*R + q*1024 + t in registers.  With parts > 1 the grid is
-// 8*parts workgroups of which every eighth participates, so on MI355X (where
-// workgroup b lands on XCD b % 8) all participants share one XCD and its L2;
-// correctness does not depend on that placement, since every exchange is an
-// agent-scope release/acquire.
+// Workgroup p of `parts` owns rows r0 + p*1024 ..., one row per thread in
+// registers.  With parts > 1 the grid is 8*parts workgroups of which every
+// eighth participates, so on MI355X (workgroup b runs on XCD b % 8) all
+// participants share one XCD and its L2; correctness does not depend on that
+// placement, every exchange being an agent-scope atomic.
 // ---------------------------------------------------------------------------
-template <typename T, int RPT>
+template <typename T>
 __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, int n, int r0, int c0, int w,
                                                    int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
-                                                   unsigned char* mb, uint32_t epoch0, int parts, int* err) {
+                                                   uint64_t* __restrict__ mb, uint32_t epoch0, int parts, int* err) {
   constexpr int W = kLeafW;
-  constexpr int R = RPT * kLeafThreads;
+  constexpr int VW = MbLayout<T>::VW;
+  constexpr int STRIDE = MbLayout<T>::STRIDE;
   const bool multi = parts > 1;
   if (multi && (blockIdx.x % kXcds) != 0) return;
   const int p = multi ? (int)(blockIdx.x / kXcds) : 0;
@@ -70,195 +88,259 @@ __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, i
 
   __shared__ T red_mag[16];
   __shared__ int red_row[16];
-  __shared__ T cand_mag[kMaxParts];
+  __shared__ __attribute__((aligned(16))) uint32_t cand_bits[kMaxParts][W * VW];
+  __shared__ __attribute__((aligned(16))) uint32_t mag_bits[kMaxParts][VW > 1 ? VW : 2];
   __shared__ int cand_row[kMaxParts];
-  __shared__ T u_s[W];  // pivot row
-  __shared__ T d_s[W];  // old diagonal row
-  __shared__ int win_row, win_slot, aborted;
-  __shared__ int info_set;
+  __shared__ __attribute__((aligned(16))) uint32_t diag_bits[W * VW];
+  __shared__ T u_s[W];
+  __shared__ T d_s[W];
+  __shared__ int win_row, win_slot, aborted, info_set;
 
   if (t == 0) {
     aborted = 0;
     info_set = (*info != 0);
   }
 
-  T v[RPT][W];
-  int rowq[RPT];
+  const int row = r0 + p * kLeafThreads + t;
+  const bool own = row < n;
+  T v[W];
+  {
+    // descriptor at this workgroup's first row: offsets stay below 1024 rows
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + (int64_t)(r0 + p * kLeafThreads) * lda + c0);
+    const uint32_t rel = (uint32_t)t * (uint32_t)lda;
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
-    rowq[q] = r0 + p * R + q * kLeafThreads + t;
-    const bool ok = rowq[q] < n;
-    const T* src = A + (int64_t)rowq[q] * lda + c0;
-#pragma unroll
-    for (int c = 0; c < W; c++) v[q][c] = (ok && c < w) ? src[c] : T(0);
+    for (int c = 0; c < W; c++) v[c] = buf_load<T>(rs, (own & (c < w)) ? (rel + c) * (uint32_t)sizeof(T) : kBufOob);
   }
   __syncthreads();
 
 #pragma unroll W
   for (int j = 0; j < W; j++) {
     if (j < w) {
-    const int gr = r0 + j;
-    const int gc = c0 + j;
-    const uint32_t epoch = epoch0 + (uint32_t)j + 1u;
-    const int parity = (int)(epoch & 1u);
+      const int gr = r0 + j;
+      const int gc = c0 + j;
+      const uint32_t epoch = epoch0 + (uint32_t)j + 1u;
+      uint64_t* par = mb + (size_t)(epoch & 1u) * MbLayout<T>::PARITY;
 
-    // 1. this thread's candidate, then the wavefront's, then the workgroup's
-    T bm = T(-1);
-    int br = INT_MAX;
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      if (rowq[q] >= gr && rowq[q] < n) {
-        T mag = fabs(v[q][j]);
-        if (!(mag == mag)) mag = T(-1);        // NaN never wins ...
-        if (rowq[q] == gr && mag < T(0)) mag = T(0);  // ... the diagonal row starts at 0
-        if (better(mag, rowq[q], bm, br)) { bm = mag; br = rowq[q]; }
+      // 1. candidate: this thread's row, then the wavefront's, then the workgroup's
+      T bm = T(-1);
+      int br = INT_MAX;
+      if (own && row >= gr) {
+        T mag = fabs(v[j]);
+        if (!(mag == mag)) mag = T(-1);           // NaN never wins ...
+        if (row == gr && mag < T(0)) mag = T(0);  // ... the diagonal row starts at 0
+        bm = mag;
+        br = row;
       }
-    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      T om = __shfl_xor(bm, off);
-      int orr = __shfl_xor(br, off);
-      if (better(om, orr, bm, br)) { bm = om; br = orr; }
-    }
-    if (lane == 0) { red_mag[wave] = bm; red_row[wave] = br; }
-    __syncthreads();
-    if (t < 64) {
-      bm = t < 16 ? red_mag[t] : T(-1);
-      br = t < 16 ? red_row[t] : INT_MAX;
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) {
+      for (int off = 32; off > 0; off >>= 1) {
         T om = __shfl_xor(bm, off);
         int orr = __shfl_xor(br, off);
         if (better(om, orr, bm, br)) { bm = om; br = orr; }
       }
-      if (t == 0) { red_mag[0] = bm; red_row[0] = br; }
-    }
-    __syncthreads();
-    const T wg_mag = red_mag[0];
-    const int wg_row = red_row[0];
-
-    if (!multi) {
-      // the pivot row and the old diagonal row straight through LDS
-#pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        if (rowq[q] == wg_row) {
-#pragma unroll
-          for (int c = 0; c < W; c++) u_s[c] = v[q][c];
-        }
-        if (rowq[q] == gr) {
-#pragma unroll
-          for (int c = 0; c < W; c++) d_s[c] = v[q][c];
-        }
-      }
-      if (t == 0) win_row = wg_row;
-      __syncthreads();
-    } else {
-      // 2. publish this workgroup's candidate (and the diagonal row if owned)
-#pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        if (rowq[q] == wg_row && wg_row != INT_MAX) {
-          Slot* s = mb_slot(mb, parity, p);
-          s->mag = (double)wg_mag;
-          s->row = wg_row;
-#pragma unroll
-          for (int c = 0; c < W; c++) s->vals[c] = (double)v[q][c];
-        }
-        if (rowq[q] == gr) {
-          Slot* s = mb_slot(mb, parity, kMaxParts);
-#pragma unroll
-          for (int c = 0; c < W; c++) s->vals[c] = (double)v[q][c];
-        }
-      }
-      if (wg_row == INT_MAX && t == 0) {
-        Slot* s = mb_slot(mb, parity, p);
-        s->mag = -1.0;
-        s->row = INT_MAX;
-      }
-      __syncthreads();  // every store of the workgroup has completed
-      if (t == 0) __hip_atomic_store(mb_flag(mb, p), epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      // 3. wait for every participant's flag, then read its candidate
-      if (t < parts) {
-        long spins = 0;
-        while ((int)(__hip_atomic_load(mb_flag(mb, t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
-          if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const Slot* s = mb_slot(mb, parity, t);
-        cand_mag[t] = (T)s->mag;
-        cand_row[t] = s->row;
-      }
+      if (lane == 0) { red_mag[wave] = bm; red_row[wave] = br; }
       __syncthreads();
       if (t < 64) {
-        T cm = t < parts ? cand_mag[t] : T(-1);
-        int cr = t < parts ? cand_row[t] : INT_MAX;
-        int cs = t;
+        bm = t < 16 ? red_mag[t] : T(-1);
+        br = t < 16 ? red_row[t] : INT_MAX;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          T om = __shfl_xor(cm, off);
-          int orr = __shfl_xor(cr, off);
-          int os = __shfl_xor(cs, off);
-          if (better(om, orr, cm, cr)) { cm = om; cr = orr; cs = os; }
+        for (int off = 8; off > 0; off >>= 1) {
+          T om = __shfl_xor(bm, off);
+          int orr = __shfl_xor(br, off);
+          if (better(om, orr, bm, br)) { bm = om; br = orr; }
         }
-        if (t == 0) { win_row = cr; win_slot = cs; }
+        if (t == 0) { red_mag[0] = bm; red_row[0] = br; win_row = br; }
       }
       __syncthreads();
-      if (t < W) {
-        const Slot* ws = mb_slot(mb, parity, win_slot);
-        const Slot* ds = mb_slot(mb, parity, kMaxParts);
-        u_s[t] = (T)ws->vals[t];
-        d_s[t] = (T)ds->vals[t];
-      }
-      __syncthreads();
-    }
+      const T wg_mag = red_mag[0];
+      const int wg_row = red_row[0];
 
-    const int wrow = (win_row == INT_MAX || aborted) ? gr : win_row;
-    // 4. interchange inside the leaf's columns, record the pivot
-    if (wrow != gr) {
+      if (!multi) {
+        if (own && row == wg_row) {
 #pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        if (rowq[q] == gr) {
+          for (int c = 0; c < W; c++) u_s[c] = v[c];
+        }
+        if (own && row == gr) {
 #pragma unroll
-          for (int c = 0; c < W; c++) v[q][c] = u_s[c];
-        } else if (rowq[q] == wrow) {
+          for (int c = 0; c < W; c++) d_s[c] = v[c];
+        }
+        __syncthreads();
+      } else {
+        // 2. publish: the winning row (or a no-candidate marker) and the diagonal row
+        uint64_t* slot = par + (size_t)p * STRIDE;
+        if (wg_row != INT_MAX) {
+          if (own && row == wg_row) {
 #pragma unroll
-          for (int c = 0; c < W; c++) v[q][c] = d_s[c];
+            for (int c = 0; c < W; c++) mb_put_value<T>(slot + 1 + VW + c * VW, v[c], epoch);
+            mb_put_value<T>(slot + 1, wg_mag, epoch);
+            mb_put(slot, (uint32_t)wg_row, epoch);  // header last: pollers watch it
+          }
+        } else {
+          if (t < W * VW) mb_put(slot + 1 + VW + t, 0u, epoch);
+          if (t == 0) {
+            mb_put_value<T>(slot + 1, T(-1), epoch);
+            mb_put(slot, (uint32_t)INT_MAX, epoch);
+          }
+        }
+        if (own && row == gr) {
+          uint64_t* dslot = par + (size_t)kMaxParts * STRIDE;
+#pragma unroll
+          for (int c = 0; c < W; c++) mb_put_value<T>(dslot + c * VW, v[c], epoch);
+        }
+        // 3. one poller per participant slot (with backoff), then one pass over
+        //    every participant's words, each still validated by its epoch
+        if (t < parts) {
+          long spins = 0;
+          while ((uint32_t)__hip_atomic_load(par + (size_t)t * STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                 epoch) {
+            if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        __syncthreads();
+        const int words = parts * STRIDE + W * VW;
+        for (int idx = t; idx < words; idx += kLeafThreads) {
+          const uint64_t* src = idx < parts * STRIDE ? par + idx : par + (size_t)kMaxParts * STRIDE + (idx - parts * STRIDE);
+          uint64_t wv;
+          long spins = 0;
+          for (;;) {
+            wv = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)wv == epoch) break;
+            if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          const uint32_t payload = (uint32_t)(wv >> 32);
+          if (idx < parts * STRIDE) {
+            const int s2 = idx / STRIDE, f = idx % STRIDE;
+            if (f == 0) cand_row[s2] = (int)payload;
+            else if (f <= VW) mag_bits[s2][f - 1] = payload;
+            else cand_bits[s2][f - 1 - VW] = payload;
+          } else {
+            diag_bits[idx - parts * STRIDE] = payload;
+          }
+        }
+        __syncthreads();
+        if (t < 64) {
+          T cm = T(-1);
+          int cr = INT_MAX, cs = t < parts ? t : 0;
+          if (t < parts) {
+            cm = *(const T*)&mag_bits[t][0];
+            cr = cand_row[t];
+          }
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) {
+            T om = __shfl_xor(cm, off);
+            int orr = __shfl_xor(cr, off);
+            int os = __shfl_xor(cs, off);
+            if (better(om, orr, cm, cr)) { cm = om; cr = orr; cs = os; }
+          }
+          if (t == 0) { win_row = cr; win_slot = cs; }
+        }
+        __syncthreads();
+        if (t < W) {
+          u_s[t] = ((const T*)&cand_bits[win_slot][0])[t];
+          d_s[t] = ((const T*)&diag_bits[0])[t];
+        }
+        __syncthreads();
+      }
+
+      const int wrow = (win_row == INT_MAX || aborted) ? gr : win_row;
+      // 4. interchange inside the leaf's columns, record the pivot
+      if (wrow != gr && own) {
+        if (row == gr) {
+#pragma unroll
+          for (int c = 0; c < W; c++) v[c] = u_s[c];
+        } else if (row == wrow) {
+#pragma unroll
+          for (int c = 0; c < W; c++) v[c] = d_s[c];
         }
       }
-    }
-    T pivot = (wrow != gr) ? u_s[j] : d_s[j];
-    if (p == 0 && t == 0) {
-      ipiv[gc] = wrow;
-      if (fabs(pivot) == T(0) && !info_set) {
-        *info = gc + 1;
-        info_set = 1;
+      const T* urow = (wrow != gr) ? u_s : d_s;  // the pivot row, LDS broadcast reads
+      T pivot = urow[j];
+      if (t == 0 && p == 0) {
+        ipiv[gc] = wrow;
+        if (fabs(pivot) == T(0) && !info_set) {
+          *info = gc + 1;
+          info_set = 1;
+        }
       }
-    }
-    if (pivot == T(0)) pivot = T(kPivotFloor);
-    // 5. multipliers and the rank-1 update of the leaf's remaining columns
-    const T* urow = (wrow != gr) ? u_s : d_s;  // LDS broadcast reads
+      if (pivot == T(0)) pivot = T(kPivotFloor);
+      // 5. multipliers and the rank-1 update of the leaf's remaining columns
+      if (own && row > gr) {
+        const T l = v[j] / pivot;
+        v[j] = l;
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      if (rowq[q] > gr && rowq[q] < n) {
-        const T l = v[q][j] / pivot;
-        v[q][j] = l;
-#pragma unroll
-        for (int c = j + 1; c < W; c++) v[q][c] = v[q][c] - l * urow[c];
+        for (int c = j + 1; c < W; c++) v[c] = v[c] - l * urow[c];
       }
-    }
-    __syncthreads();
     }
   }
 
   if (aborted && t == 0) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
+  if (own) {
+    T* dst = A + (int64_t)row * lda + c0;
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
-    if (rowq[q] < n) {
-      T* dst = A + (int64_t)rowq[q] * lda + c0;
-#pragma unroll
-      for (int c = 0; c < W; c++)
-        if (c < w) dst[c] = v[q][c];
-    }
+    for (int c = 0; c < W; c++)
+      if (c < w) dst[c] = v[c];
   }
+}
+
+// Replay the interchanges ipiv[k1..k2) (k2 - k1 <= 32) over the columns
+// [col_lo, col_hi) minus [skip_lo, skip_lo + skip_w): laswp / laswp_matrix
+// (numax/linalg/panel.mojo:1824-1852, :2279-2313).  The net effect of the
+// sequence on the <= 2(k2-k1) rows it touches is worked out once per
+// workgroup in LDS; each thread then moves one column with all its loads in
+// flight together.
+template <typename T>
+__global__ void __launch_bounds__(256) laswp_gather_kernel(T* __restrict__ A, int lda, int col_lo, int ncols,
+                                                           int skip_lo, int skip_w, const int32_t* __restrict__ ipiv,
+                                                           int k1, int k2) {
+  __shared__ int mv_dst[2 * kLeafW], mv_src[2 * kLeafW], mv_count;
+  const int t = threadIdx.x;
+  const int w = k2 - k1;
+  if (t < 64) {
+    // wavefront 0: lane e tracks involved row e and the original row now in it
+    int myrow = (t < w) ? k1 + t : -1;
+    int mine = myrow;
+    int nr = w;
+    for (int j = 0; j < w; j++) {
+      const int q = ipiv[k1 + j];
+      const unsigned long long hit = __ballot(t < nr && myrow == q);
+      int pos;
+      if (hit) {
+        pos = __ffsll((long long)hit) - 1;
+      } else {
+        pos = nr;
+        if (t == nr) { myrow = q; mine = q; }
+        nr++;
+      }
+      const int cj = __shfl(mine, j);
+      const int cp = __shfl(mine, pos);
+      if (t == j) mine = cp;
+      if (t == pos) mine = cj;
+    }
+    const bool moved = (t < nr) && (mine != myrow);
+    const unsigned long long mm = __ballot(moved);
+    if (moved) {
+      const int at = __popcll(mm & ((1ull << t) - 1ull));
+      mv_dst[at] = myrow;
+      mv_src[at] = mine;
+    }
+    if (t == 0) mv_count = __popcll(mm);
+  }
+  __syncthreads();
+  const int moves = mv_count;
+  const int i = blockIdx.x * blockDim.x + t;
+  if (moves == 0 || i >= ncols) return;
+  int c = col_lo + i;
+  if (c >= skip_lo) c += skip_w;
+  T vals[2 * kLeafW];
+#pragma unroll
+  for (int e = 0; e < 2 * kLeafW; e++) {
+    // unconditional loads (a clamped, valid row) so all are in flight together
+    vals[e] = A[(int64_t)mv_src[e < moves ? e : moves - 1] * lda + c];
+  }
+#pragma unroll
+  for (int e = 0; e < 2 * kLeafW; e++)
+    if (e < moves) A[(int64_t)mv_dst[e] * lda + c] = vals[e];
 }
 
 // Replay interchanges k1..k2-1 (in order) over the columns [col_lo, col_hi)
@@ -398,8 +480,98 @@ __global__ void zero_i32(int32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// inv(op(T)) for n <= 64 into a dense 64 x 64 (ld 64) block: one thread per
+// column of the inverse, substitution against the triangle staged in LDS.
+template <typename T, int UPPER, int UNIT, int TRANS>
+__global__ void __launch_bounds__(1024) tri_inv_small(int n, const T* __restrict__ Tm, int ldt, T* __restrict__ inv) {
+  // Gauss-Jordan sweep on [op(T) | I]: one pivot row per step, every other
+  // row updated in parallel (n <= 64 steps, each a barrier apart).
+  __shared__ T S[64][65];
+  __shared__ T X[64][65];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < 64 * 64; idx += 1024) {
+    const int i = idx / 64, q = idx % 64;
+    S[i][q] = (i < n && q < n) ? (TRANS ? Tm[(int64_t)q * ldt + i] : Tm[(int64_t)i * ldt + q]) : T(0);
+    X[i][q] = (i == q) ? T(1) : T(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < n; s++) {
+    const int j = UPPER ? n - 1 - s : s;
+    if (!UNIT) {
+      if (t < 64) X[j][t] = X[j][t] / S[j][j];
+      __syncthreads();
+    }
+    for (int idx = t; idx < 64 * 64; idx += 1024) {
+      const int r = idx / 64, c = idx % 64;
+      const bool below = UPPER ? (r < j) : (r > j && r < n);
+      if (below) X[r][c] = X[r][c] - S[r][j] * X[j][c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < 64 * 64; idx += 1024) {
+    const int i = idx / 64, q = idx % 64;
+    inv[idx] = (i < n && q < n) ? X[i][q] : T(0);
+  }
+}
+
+// Apply a 64 x 64 (ld 64) inverse to 64 vectors per workgroup, in place:
+// each vector x of B becomes inv x.  Thread t owns vector t % 64 and 16 of
+// its n entries; the inverse and the vectors are staged in LDS.
+template <typename T, int VEC_ROWS>
+__global__ void __launch_bounds__(256) tri_apply_small(int n, int nvec, const T* __restrict__ inv,
+                                                       T* __restrict__ B, int ldb) {
+  __shared__ T Is[64][65];
+  __shared__ T Xs[64][65];
+  const int t = threadIdx.x;
+  const int v0 = blockIdx.x * 64;
+  for (int idx = t; idx < 64 * 64; idx += 256) Is[idx / 64][idx % 64] = inv[idx];
+  if (VEC_ROWS) {
+    for (int idx = t; idx < n * 64; idx += 256) {
+      int v = idx / n, i = idx % n;
+      if (v0 + v < nvec) Xs[i][v] = B[(int64_t)(v0 + v) * ldb + i];
+    }
+  } else {
+    for (int idx = t; idx < n * 64; idx += 256) {
+      int i = idx / 64, v = idx % 64;
+      if (v0 + v < nvec) Xs[i][v] = B[(int64_t)i * ldb + v0 + v];
+    }
+  }
+  __syncthreads();
+  const int v = t & 63, i0 = (t >> 6) * 16;
+  T y[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) y[r] = T(0);
+  for (int q = 0; q < n; q++) {
+    const T xq = Xs[q][v];
+#pragma unroll
+    for (int r = 0; r < 16; r++) y[r] += Is[i0 + r][q] * xq;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) Xs[i0 + r][v] = y[r];
+  __syncthreads();
+  if (VEC_ROWS) {
+    for (int idx = t; idx < n * 64; idx += 256) {
+      int vv = idx / n, i = idx % n;
+      if (v0 + vv < nvec) B[(int64_t)(v0 + vv) * ldb + i] = Xs[i][vv];
+    }
+  } else {
+    for (int idx = t; idx < n * 64; idx += 256) {
+      int i = idx / 64, vv = idx % 64;
+      if (v0 + vv < nvec) B[(int64_t)i * ldb + v0 + vv] = Xs[i][vv];
+    }
+  }
+}
+
 template <typename T, int UPPER, int UNIT, int TRANS, int VEC_ROWS>
 void launch_base(nmx_ctx* ctx, int n, int nvec, const T* Tm, int ldt, T* B, int ldb) {
+  if (nvec > 128 && ctx->tri_inv) {
+    T* inv = (T*)ctx->tri_inv;
+    ProfScope ps(ctx, K_TRSM, 2.0 * n * (double)n * nvec, sizeof(T) * (2.0 * n * nvec + 64.0 * 64));
+    tri_inv_small<T, UPPER, UNIT, TRANS><<<1, 1024, 0, ctx->stream>>>(n, Tm, ldt, inv);
+    tri_apply_small<T, VEC_ROWS><<<ceil_div(nvec, 64), 256, 0, ctx->stream>>>(n, nvec, inv, B, ldb);
+    return;
+  }
   ProfScope ps(ctx, K_TRSM, (double)n * n * nvec, sizeof(T) * ((double)n * n + 2.0 * n * nvec));
   trsm_base<T, UPPER, UNIT, TRANS, VEC_ROWS><<<ceil_div(nvec, 64), 64, 0, ctx->stream>>>(n, nvec, Tm, ldt, B, ldb);
 }
@@ -495,40 +667,38 @@ int laswp_launch(nmx_ctx* ctx, T* A, int lda, int col_lo, int col_hi, int skip_l
   int ncols = col_hi - col_lo - skip_w;
   if (ncols <= 0) return NMX_OK;
   ProfScope ps(ctx, K_LASWP, 0.0, 4.0 * sizeof(T) * (double)ncols * (k2 - k1));
-  laswp_kernel<T><<<ceil_div(ncols, 256), 256, 0, ctx->stream>>>(A, lda, col_lo, ncols, s_lo, skip_w, ipiv, k1, k2);
+  if (k2 - k1 <= kLeafW)
+    laswp_gather_kernel<T><<<ceil_div(ncols, 256), 256, 0, ctx->stream>>>(A, lda, col_lo, ncols, s_lo, skip_w, ipiv, k1,
+                                                                           k2);
+  else
+    laswp_kernel<T><<<ceil_div(ncols, 256), 256, 0, ctx->stream>>>(A, lda, col_lo, ncols, s_lo, skip_w, ipiv, k1, k2);
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
 
 template <typename T>
-int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info) {
+int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info,
+                      int swap_others) {
   const int m = n - r0;
   if (m <= 0 || w <= 0) return NMX_OK;
   if (w > kLeafW) return NMX_ERR_UNSUPPORTED;
-  int rpt = 1;
   int parts = ceil_div(m, kLeafThreads);
-  if (parts > kMaxParts) {
-    if (sizeof(T) == 8) return NMX_ERR_UNSUPPORTED;  // FP64 leaf holds one row per thread
-    rpt = 2;
-    parts = ceil_div(m, 2 * kLeafThreads);
-    if (parts > kMaxParts) return NMX_ERR_UNSUPPORTED;
-  }
+  if (parts > kMaxParts) return NMX_ERR_UNSUPPORTED;  // n - r0 <= 32768 rows
   uint32_t epoch0 = 0;
   NMX_TRY(nmx_epoch_reserve(ctx, (uint32_t)w, &epoch0));
-  unsigned char* mb = (unsigned char*)ctx->mailbox;
+  uint64_t* mb = (uint64_t*)ctx->mailbox;
   int* err = ctx->err;
-  ProfScope ps(ctx, K_LEAF, (double)m * w * w, 2.0 * sizeof(T) * (double)m * w);
-  if (parts == 1) {
-    if (rpt == 1)
-      getf2_leaf<T, 1><<<1, kLeafThreads, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
-    else
-      getf2_leaf<T, 2><<<1, kLeafThreads, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
+  {
+    ProfScope ps(ctx, K_LEAF, (double)m * w * w, 2.0 * sizeof(T) * (double)m * w);
+    // At most 8 x 32 workgroups of 1024 threads: one per CU of an otherwise
+    // idle device (the stream orders this launch after the previous kernel).
+    // A participant that is not resident makes the others' bounded waits
+    // expire, which the context reports as NMX_ERR_SYNC_TIMEOUT.
+    const int grid = parts == 1 ? 1 : parts * kXcds;
+    getf2_leaf<T><<<grid, kLeafThreads, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
     NMX_HIP(ctx, hipGetLastError());
-    return NMX_OK;
   }
-  void* args[] = {&A, &lda, (void*)&n, &r0, &c0, &w, &ipiv, &info, &mb, &epoch0, &parts, &err};
-  const void* fn = rpt == 1 ? (const void*)getf2_leaf<T, 1> : (const void*)getf2_leaf<T, 2>;
-  NMX_HIP(ctx, hipLaunchCooperativeKernel(fn, dim3(parts * kXcds), dim3(kLeafThreads), args, 0, ctx->stream));
+  if (swap_others) return laswp_launch<T>(ctx, A, lda, 0, n, c0, c0 + w, ipiv, r0, r0 + w);
   return NMX_OK;
 }
 
@@ -568,7 +738,7 @@ int tril_launch(nmx_ctx* ctx, int n, T* A, int lda) {
   template int trsm_left<T>(nmx_ctx*, int, int, int, int, int, const T*, int, T*, int);                            \
   template int trsm_right_lower_t<T>(nmx_ctx*, int, int, const T*, int, T*, int);                                  \
   template int laswp_launch<T>(nmx_ctx*, T*, int, int, int, int, int, const int32_t*, int, int);                   \
-  template int getf2_leaf_launch<T>(nmx_ctx*, int, T*, int, int, int, int, int32_t*, int32_t*);                    \
+  template int getf2_leaf_launch<T>(nmx_ctx*, int, T*, int, int, int, int, int32_t*, int32_t*, int);               \
   template int diag_launch<T>(nmx_ctx*, int, const T*, int, T*);                                                   \
   template int potrf_diag_launch<T>(nmx_ctx*, T*, int, int, int, int32_t*);                                        \
   template int tril_launch<T>(nmx_ctx*, int, T*, int);
