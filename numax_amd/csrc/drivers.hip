@@ -130,6 +130,22 @@ NMX_API int nmx_getrf_f32(nmx_ctx* ctx, int n, float* A, int lda, int32_t* ipiv,
   return getrf_api<float>(ctx, n, A, lda, ipiv, info, block, base);
 }
 
+// laswp over all n pivots (panel.mojo:1824-1852): the net permutation is
+// formed once and applied as one gather through the workspace.
+template <typename T>
+static int laswp_full(nmx_ctx* ctx, int n, int nrhs, T* B, int ldb, const int32_t* ipiv) {
+  void* ws = nullptr;
+  const size_t vec_bytes = ((sizeof(T) * (size_t)n * nrhs + 255) / 256) * 256;
+  NMX_TRY(nmx_workspace(ctx, vec_bytes + sizeof(int32_t) * (size_t)n, &ws));
+  T* W = (T*)ws;
+  int32_t* perm = (int32_t*)((char*)ws + vec_bytes);
+  NMX_TRY(perm_from_ipiv(ctx, n, ipiv, perm));
+  NMX_TRY((gather_rows<T, T>(ctx, n, nrhs, perm, B, ldb, W, nrhs)));
+  NMX_HIP(ctx, hipMemcpy2DAsync(B, sizeof(T) * (size_t)ldb, W, sizeof(T) * (size_t)nrhs, sizeof(T) * (size_t)nrhs, n,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  return NMX_OK;
+}
+
 // LU.solve (lu.mojo:181-354): laswp, forward (unit) then back substitution.
 template <typename T>
 static int getrs_api(nmx_ctx* ctx, int n, int nrhs, const T* LU, int lda, const int32_t* ipiv, T* B, int ldb) {
@@ -142,7 +158,7 @@ static int getrs_api(nmx_ctx* ctx, int n, int nrhs, const T* LU, int lda, const 
   if (n > 0 && nrhs > 0 && !B) return -7;
   if (ldb < (nrhs > 1 ? nrhs : 1)) return -8;
   if (n == 0 || nrhs == 0) return NMX_OK;
-  NMX_TRY(laswp_launch<T>(ctx, B, ldb, 0, nrhs, 0, 0, ipiv, 0, n));
+  NMX_TRY(laswp_full<T>(ctx, n, nrhs, B, ldb, ipiv));
   NMX_TRY(trsm_left<T>(ctx, 0, 1, 0, n, nrhs, LU, lda, B, ldb));
   return trsm_left<T>(ctx, 1, 0, 0, n, nrhs, LU, lda, B, ldb);
 }
@@ -164,7 +180,7 @@ NMX_API int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, con
   if (n == 0 || nrhs == 0) return NMX_OK;
   if (!B) return -4;
   if (!ipiv) return -6;
-  return laswp_launch<double>(ctx, B, ldb, 0, nrhs, 0, 0, ipiv, 0, n);
+  return laswp_full<double>(ctx, n, nrhs, B, ldb, ipiv);
 }
 
 template <typename T>

@@ -517,29 +517,64 @@ __global__ void __launch_bounds__(256) tri_inv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 constexpr long kTrsvSpin = 1L << 23;
 
-// acc[r] (r = wave + 4 q) += sum_c tile[r][c] * v[c] over a 128 x 128 tile,
-// each lane taking columns lane and lane + 64 (coalesced row reads); the
-// per-lane partial sums are reduced across the wavefront only at the end.
-__device__ __forceinline__ void tile_gemv_accum(float s[32], const float* __restrict__ tile, int64_t ld, int rows,
-                                                int cols, const float* v, int wave, int lane) {
-  // all 64 loads of a lane in flight at once: buffer loads, out-of-range
-  // elements read 0 (v holds 0 past `cols`)
+// A 128 x 128 tile as two register slabs per lane: with 8 wavefronts, row
+// r = wave + 8 q (q < 16), columns lane and lane + 64 (coalesced row reads).
+// All 32 loads of a lane are in flight at once; rows >= rows or columns
+// >= cols read 0.
+constexpr int kTrsvThreads = 512, kTrsvWaves = kTrsvThreads / 64, kTrsvRows = TB / kTrsvWaves;
+struct TileRegs {
+  float a0[kTrsvRows], a1[kTrsvRows];
+};
+
+__device__ __forceinline__ void tile_load(TileRegs& T, const float* __restrict__ tile, int64_t ld, int rows, int cols,
+                                          int wave, int lane) {
   const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tile);
-  const float v0 = v[lane], v1 = v[lane + 64];
-  float a0[32], a1[32];
+  // The empty asm makes the lane offset opaque, so the offsets below are
+  // recomputed per tile instead of being hoisted out of the caller's loop.
+  uint32_t vb = ((uint32_t)wave * (uint32_t)ld + (uint32_t)lane) * 4u;
+  asm volatile("" : "+v"(vb));
+  const bool c0 = lane < cols, c1 = lane + 64 < cols;
+  const uint32_t rstep = 4u * kTrsvWaves * (uint32_t)ld;  // kTrsvWaves rows, in bytes
 #pragma unroll
-  for (int q = 0; q < 32; q++) {
-    const int r = wave + 4 * q;
-    const uint32_t base = (uint32_t)r * (uint32_t)ld;
-    a0[q] = buf_load<float>(rt, ((r < rows) & (lane < cols)) ? (base + lane) * 4u : kBufOob);
-    a1[q] = buf_load<float>(rt, ((r < rows) & (lane + 64 < cols)) ? (base + lane + 64) * 4u : kBufOob);
+  for (int q = 0; q < kTrsvRows; q++) {
+    const bool rv = wave + kTrsvWaves * q < rows;
+    const uint32_t off = vb + (uint32_t)q * rstep;
+    T.a0[q] = buf_load<float>(rt, (rv & c0) ? off : kBufOob);
+    T.a1[q] = buf_load<float>(rt, (rv & c1) ? off + 256u : kBufOob);
   }
+}
+
+// s[q] += tile[wave + 8 q][.] . v  (per-lane partial sums)
+__device__ __forceinline__ void tile_fma(float s[kTrsvRows], const TileRegs& T, const float* v, int lane) {
+  const float v0 = v[lane], v1 = v[lane + 64];
 #pragma unroll
-  for (int q = 0; q < 32; q++) s[q] += a0[q] * v0 + a1[q] * v1;
+  for (int q = 0; q < kTrsvRows; q++) s[q] += T.a0[q] * v0 + T.a1[q] * v1;
+}
+
+// Sum N per-lane partials over the wavefront by recursive halving
+// (reduce-scatter, N - 1 + log2(64 / N) exchanges instead of 6 N).  Lane l
+// ends with the total of partial l / (64 / N).
+template <int N>
+__device__ __forceinline__ float wave_reduce(float s[N], int lane) {
+  int m = 32;
+#pragma unroll
+  for (int w = N / 2; w >= 1; w >>= 1, m >>= 1) {
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int k = 0; k < w; k++) {
+      const float keep = hi ? s[w + k] : s[k];
+      const float send = hi ? s[k] : s[w + k];
+      s[k] = keep + __shfl_xor(send, m);
+    }
+  }
+  float v = s[0];
+#pragma unroll
+  for (; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
 }
 
 template <int UPPER>
-__global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* __restrict__ F, int ldf,
+__global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const float* __restrict__ F, int ldf,
                                                            const float* __restrict__ inv, float* __restrict__ x,
                                                            uint64_t* __restrict__ xpub, uint32_t epoch,
                                                            int* __restrict__ err, uint64_t* __restrict__ stamps) {
@@ -547,8 +582,10 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* _
   // last block first), and a block only waits for blocks dispatched before
   // it, so the chain needs no co-residency guarantee.  Each solved entry is
   // published as one 64-bit word (value bits, epoch): a reader validates
-  // every word it uses on its own, with no fence.
-  __shared__ float xs[TB];
+  // every word it uses on its own, with no fence.  The strip's tiles and the
+  // diagonal inverse do not depend on the solution, so they are loaded one
+  // step ahead of the wait for the block they multiply.
+  __shared__ float xs[2][TB];
   __shared__ float rhs[TB];
   __shared__ int aborted;
   const int nblk = (n + TB - 1) / TB;
@@ -556,63 +593,78 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(int n, const float* _
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int k0 = bi * TB, nb = min(TB, n - k0);
   if (t == 0) aborted = 0;
-  float s[32];
+  float s[kTrsvRows];
 #pragma unroll
-  for (int q = 0; q < 32; q++) s[q] = 0.f;
-  __syncthreads();
+  for (int q = 0; q < kTrsvRows; q++) s[q] = 0.f;
+  TileRegs Ia, Ib;
   const int steps = UPPER ? nblk - 1 - bi : bi;
-  for (int st = 0; st < steps; st++) {
-    const int bj = UPPER ? nblk - 1 - st : st;
-    const int j0 = bj * TB, jb = min(TB, n - j0);
-    // one poller per workgroup (with backoff) keeps the spinning off the
-    // producer's cache lines; then every thread reads and validates its word
-    if (t == 0) {
-      long spins = 0;
-      while ((uint32_t)__hip_atomic_load(&xpub[j0 + jb - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
-        __builtin_amdgcn_s_sleep(1);
+  auto col_block = [&](int st) { return UPPER ? nblk - 1 - st : st; };
+  auto fetch = [&](TileRegs& T, int st) {
+    const int j0 = col_block(st) * TB;
+    tile_load(T, F + (int64_t)k0 * ldf + j0, ldf, nb, min(TB, n - j0), wave, lane);
+  };
+  // wait for block st's solution, stage it in xs[st & 1]
+  auto take = [&](int st) {
+    const int j0 = col_block(st) * TB, jb = min(TB, n - j0);
+    float* dst = xs[st & 1];
+    if (t < TB) {
+      float v = 0.f;
+      if (t < jb) {
+        uint64_t wv;
+        long spins = 0;
+        for (;;) {
+          wv = __hip_atomic_load(&xpub[j0 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((uint32_t)wv == epoch) break;
+          if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        v = __uint_as_float((uint32_t)(wv >> 32));
       }
+      dst[t] = v;
     }
     __syncthreads();
-    if (t >= jb && t < TB) xs[t] = 0.f;
-    if (t < jb) {
-      uint64_t wv;
-      long spins = 0;
-      for (;;) {
-        wv = __hip_atomic_load(&xpub[j0 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)wv == epoch) break;
-        if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      xs[t] = __uint_as_float((uint32_t)(wv >> 32));
-    }
-    __syncthreads();
-    tile_gemv_accum(s, F + (int64_t)k0 * ldf + j0, ldf, nb, jb, xs, wave, lane);
-    __syncthreads();
-  }
-  // reduce the partial sums across the wavefront: rhs = b - L_i,<i x_<i
-#pragma unroll
-  for (int q = 0; q < 32; q++) {
-    float v = s[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    const int r = wave + 4 * q;
-    if (lane == 0 && r < TB) rhs[r] = (r < nb) ? x[k0 + r] - v : 0.f;
-  }
+  };
   __syncthreads();
-  // x_i = inv_ii rhs, stored plain and published tagged
+  if (steps > 0) fetch(Ia, 0);
+  int st = 0;
+  for (; st + 1 < steps; st += 2) {
+    fetch(Ib, st + 1);
+    take(st);
+    tile_fma(s, Ia, xs[st & 1], lane);
+    if (st + 2 < steps) fetch(Ia, st + 2);
+    take(st + 1);
+    tile_fma(s, Ib, xs[(st + 1) & 1], lane);
+  }
+  // rhs = b - L_i,<i x_<i, then x_i = inv_ii rhs, stored plain and
+  // published tagged
+  auto finish = [&](const TileRegs& Inv) {
+    constexpr int kSpread = 64 / kTrsvRows;  // lanes holding one row's total
+    const int r = wave + kTrsvWaves * (lane / kSpread);
+    {
+      const float v = wave_reduce<kTrsvRows>(s, lane);
+      if (lane % kSpread == 0) rhs[r] = (r < nb) ? x[k0 + r] - v : 0.f;
+    }
+    __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 32; q++) s[q] = 0.f;
-  tile_gemv_accum(s, inv + (int64_t)bi * TB * TB, TB, nb, nb, rhs, wave, lane);
-#pragma unroll
-  for (int q = 0; q < 32; q++) {
-    float v = s[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    const int r = wave + 4 * q;
-    if (lane == 0 && r < nb) {
+    for (int q = 0; q < kTrsvRows; q++) s[q] = 0.f;
+    tile_fma(s, Inv, rhs, lane);
+    const float v = wave_reduce<kTrsvRows>(s, lane);
+    if (lane % kSpread == 0 && r < nb) {
       x[k0 + r] = v;
       __hip_atomic_store(&xpub[k0 + r], ((uint64_t)__float_as_uint(v) << 32) | epoch, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
+  };
+  // the diagonal inverse rides in whichever slab is free (a uniform branch,
+  // so both slabs stay in registers)
+  if (st < steps) {
+    tile_load(Ib, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
+    take(st);
+    tile_fma(s, Ia, xs[st & 1], lane);
+    finish(Ib);
+  } else {
+    tile_load(Ia, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
+    finish(Ia);
   }
   if (t == 0 && aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
   if (stamps && t == 0) stamps[bi] = __builtin_amdgcn_s_memrealtime();  // 100 MHz clock
@@ -681,11 +733,6 @@ __global__ void cast_f64_f32(int n, const double* __restrict__ A, int lda, float
   F[(int64_t)r * ldf + c] = (float)A[(int64_t)r * lda + c];
 }
 
-__global__ void vec_f64_f32(int n, const double* __restrict__ v, float* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (float)v[i];
-}
-
 __global__ void vec_f32_add_f64(int n, const float* __restrict__ y, double* __restrict__ x, int accumulate) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = accumulate ? x[i] + (double)y[i] : (double)y[i];
@@ -700,6 +747,7 @@ struct MpFactor {
   int n = 0, block = 256, prec = 2;  // prec: 0 fp16, 1 bf16, 2 fp16 split, 3 fp32
   int pivot = 1;                     // 1: partial pivoting (GEPP); 0: butterflies, no pivoting
   int32_t* ipiv = nullptr;
+  int32_t* perm = nullptr;           // net row permutation of ipiv
   float* F = nullptr;                // n x n FP32 factors (of W^T A V without pivoting)
   float* dw = nullptr;               // 2n butterfly diagonals for W, then V
   float* dv = nullptr;
@@ -770,7 +818,7 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   NMX_TRY(grow(ctx, b->F, b->F_elems, (size_t)n * n, sizeof(float)));
   NMX_TRY(grow(ctx, b->d, b->d_elems, 4 * (size_t)n + 16, sizeof(float)));
   NMX_TRY(grow(ctx, b->inv, b->inv_elems, (size_t)nblk * TB * TB * 2, sizeof(float)));
-  NMX_TRY(grow(ctx, b->ipiv, b->piv_elems, (size_t)n, sizeof(int32_t)));
+  NMX_TRY(grow(ctx, b->ipiv, b->piv_elems, 2 * (size_t)n, sizeof(int32_t)));
   NMX_TRY(grow(ctx, b->lo, b->lo_bytes, lo_bytes, 1));
   if (!b->info) NMX_TRY(grow(ctx, b->info, unused, 64, sizeof(int32_t)));
   if (!b->maxbits) NMX_TRY(grow(ctx, b->maxbits, unused2, 64, sizeof(unsigned)));
@@ -786,6 +834,7 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.invL = b->inv;
   f.invU = b->inv + (size_t)nblk * TB * TB;
   f.ipiv = b->ipiv;
+  f.perm = b->ipiv + n;
   f.info = b->info;
   f.lo_buf = b->lo;
   f.maxbits = b->maxbits;
@@ -881,6 +930,7 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
     if (stop_after_transform) return zero_info(ctx, f.info);
     NMX_TRY(getrf_impl_ex<float>(ctx, n, f.F, n, f.ipiv, f.info, f.block, kLeafW,
                                  f.prec == 3 ? nullptr : mp_update_hook, &f));
+    NMX_TRY(perm_from_ipiv(ctx, n, f.ipiv, f.perm));
     const int nblk = ceil_div(n, TB);
     tri_inv_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL);
     tri_inv_kernel<1><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invU);
@@ -929,9 +979,7 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   const int n = f.n, n4 = n & ~3, h = n4 / 4;
   const int64_t work = (int64_t)h + (n - n4);
   if (f.pivot) {
-    vec_f64_f32<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, r, tmp);
-    NMX_HIP(ctx, hipGetLastError());
-    NMX_TRY(laswp_launch<float>(ctx, tmp, 1, 0, 1, 0, 0, f.ipiv, 0, n));
+    NMX_TRY((gather_rows<double, float>(ctx, n, 1, f.perm, r, 1, tmp, 1)));
   } else {
     rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, r, 1, tmp, 1, f.dw);
     NMX_HIP(ctx, hipGetLastError());
@@ -941,9 +989,9 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
     f.epoch += 1;
     ProfScope ps(ctx, K_MP_TRSV, 2.0 * (double)n * n / 2, 4.0 * (double)n * n / 2);
     if (up)
-      trsv_blocked_kernel<1><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invU, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+      trsv_blocked_kernel<1><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invU, tmp, f.xpub, f.epoch, ctx->err, nullptr);
     else
-      trsv_blocked_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+      trsv_blocked_kernel<0><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invL, tmp, f.xpub, f.epoch, ctx->err, nullptr);
     NMX_HIP(ctx, hipGetLastError());
   }
   if (f.pivot) {

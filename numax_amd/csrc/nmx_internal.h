@@ -122,6 +122,13 @@ template <typename T>
 int laswp_launch(nmx_ctx* ctx, T* A, int lda, int col_lo, int col_hi, int skip_lo, int skip_hi, const int32_t* ipiv,
                  int k1, int k2);
 
+// perm[i] = original row that the interchanges ipiv[0..n) bring to row i;
+// gather_rows applies it (out[i] = in[perm[i]], with a type conversion).
+constexpr size_t kPermLdsBytes = 64 * 1024;
+int perm_from_ipiv(nmx_ctx* ctx, int n, const int32_t* ipiv, int32_t* perm);
+template <typename Ti, typename To>
+int gather_rows(nmx_ctx* ctx, int n, int ncols, const int32_t* perm, const Ti* in, int ldi, To* out, int ldo);
+
 template <typename T>
 int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info,
                       int swap_others);

@@ -54,6 +54,14 @@ __device__ __forceinline__ void mb_put_value(uint64_t* w, T v, uint32_t epoch) {
   }
 }
 
+template <typename T, int N>
+__device__ __forceinline__ void rotate_left(T (&v)[N]) {
+  const T first = v[0];
+#pragma unroll
+  for (int c = 0; c + 1 < N; c++) v[c] = v[c + 1];
+  v[N - 1] = first;
+}
+
 // ---------------------------------------------------------------------------
 // Cooperative leaf of the recursive LU panel.
 //
@@ -113,9 +121,14 @@ __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, i
   }
   __syncthreads();
 
-#pragma unroll W
-  for (int j = 0; j < W; j++) {
-    if (j < w) {
+  // The column loop stays rolled (a fully unrolled one is ~230 KB of code,
+  // far beyond the instruction cache, and every column would refetch it).
+  // To keep register indices static, the row held in v[] is rotated left by
+  // one after each column: at step j, v[0] is column j, v[1 .. w-j) are the
+  // columns still to update, and the finished ones sit at the end.  After
+  // the last column the rotation is completed to a full turn.
+  for (int j = 0; j < w; j++) {
+    {
       const int gr = r0 + j;
       const int gc = c0 + j;
       const uint32_t epoch = epoch0 + (uint32_t)j + 1u;
@@ -125,7 +138,7 @@ __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, i
       T bm = T(-1);
       int br = INT_MAX;
       if (own && row >= gr) {
-        T mag = fabs(v[j]);
+        T mag = fabs(v[0]);
         if (!(mag == mag)) mag = T(-1);           // NaN never wins ...
         if (row == gr && mag < T(0)) mag = T(0);  // ... the diagonal row starts at 0
         bm = mag;
@@ -255,7 +268,7 @@ __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, i
         }
       }
       const T* urow = (wrow != gr) ? u_s : d_s;  // the pivot row, LDS broadcast reads
-      T pivot = urow[j];
+      T pivot = urow[0];
       if (t == 0 && p == 0) {
         ipiv[gc] = wrow;
         if (fabs(pivot) == T(0) && !info_set) {
@@ -266,13 +279,17 @@ __global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, i
       if (pivot == T(0)) pivot = T(kPivotFloor);
       // 5. multipliers and the rank-1 update of the leaf's remaining columns
       if (own && row > gr) {
-        const T l = v[j] / pivot;
-        v[j] = l;
+        const T l = v[0] / pivot;
+        v[0] = l;
+        const int rem = w - j;
 #pragma unroll
-        for (int c = j + 1; c < W; c++) v[c] = v[c] - l * urow[c];
+        for (int c = 1; c < W; c++) v[c] = (c < rem) ? v[c] - l * urow[c] : v[c];
       }
+      // u_s / d_s are next written after two barriers of the next column
+      rotate_left(v);
     }
   }
+  for (int j = w; j < W; j++) rotate_left(v);
 
   if (aborted && t == 0) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
   if (own) {
@@ -362,6 +379,56 @@ __global__ void laswp_kernel(T* __restrict__ A, int lda, int col_lo, int ncols, 
       A[(int64_t)o * lda + c] = a;
     }
   }
+}
+
+// Net permutation of the interchange sequence ipiv[0..n): row i of the
+// permuted matrix is row perm[i] of the original one, which is what laswp
+// over all n pivots (panel.mojo:1824-1852) leaves behind.  The sequence is a
+// dependent chain, so one thread replays it on an identity vector; in LDS a
+// step costs a few dozen cycles instead of the two dependent global
+// round trips of a swap in place.  Pivot values are clamped to [0, n), so a
+// corrupt ipiv yields a wrong permutation but no out-of-range access.
+template <int IN_LDS>
+__global__ void __launch_bounds__(1024) perm_kernel(int n, const int32_t* __restrict__ ipiv,
+                                                     int32_t* __restrict__ perm) {
+  extern __shared__ int s_perm[];
+  int* s = IN_LDS ? s_perm : perm;
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += blockDim.x) s[i] = i;
+  __syncthreads();
+  if (t == 0) {
+    const auto rs = buf_rsrc(ipiv);
+    for (int j0 = 0; j0 < n; j0 += 16) {
+      int q[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        q[u] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs, j0 + u < n ? 4u * (uint32_t)(j0 + u) : kBufOob, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int j = j0 + u;
+        if (j < n) {
+          const int p = min(max(q[u], 0), n - 1);
+          const int a = s[j], b = s[p];
+          s[j] = b;
+          s[p] = a;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (IN_LDS)
+    for (int i = t; i < n; i += blockDim.x) perm[i] = s[i];
+}
+
+// out[i][c] = in[perm[i]][c] for c < ncols, converting Ti to To.
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(256) gather_rows_kernel(int n, int ncols, const int32_t* __restrict__ perm,
+                                                          const Ti* __restrict__ in, int ldi, To* __restrict__ out,
+                                                          int ldo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * ncols) return;
+  const int i = (int)(idx / ncols), c = (int)(idx % ncols);
+  out[(int64_t)i * ldo + c] = (To)in[(int64_t)perm[i] * ldi + c];
 }
 
 // Base triangular solve for n <= 64: one thread per vector, the triangle and
@@ -675,6 +742,32 @@ int laswp_launch(nmx_ctx* ctx, T* A, int lda, int col_lo, int col_hi, int skip_l
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
+
+int perm_from_ipiv(nmx_ctx* ctx, int n, const int32_t* ipiv, int32_t* perm) {
+  if (n <= 0) return NMX_OK;
+  ProfScope ps(ctx, K_LASWP, 0.0, 8.0 * n);
+  const size_t lds = sizeof(int32_t) * (size_t)n;
+  if (lds <= kPermLdsBytes)
+    perm_kernel<1><<<1, 1024, lds, ctx->stream>>>(n, ipiv, perm);
+  else
+    perm_kernel<0><<<1, 1024, 0, ctx->stream>>>(n, ipiv, perm);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+template <typename Ti, typename To>
+int gather_rows(nmx_ctx* ctx, int n, int ncols, const int32_t* perm, const Ti* in, int ldi, To* out, int ldo) {
+  const int64_t total = (int64_t)n * ncols;
+  if (total <= 0) return NMX_OK;
+  ProfScope ps(ctx, K_LASWP, 0.0, (double)total * (sizeof(Ti) + sizeof(To)) + 4.0 * n);
+  gather_rows_kernel<Ti, To><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(n, ncols, perm, in, ldi, out,
+                                                                                       ldo);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+template int gather_rows<double, double>(nmx_ctx*, int, int, const int32_t*, const double*, int, double*, int);
+template int gather_rows<float, float>(nmx_ctx*, int, int, const int32_t*, const float*, int, float*, int);
+template int gather_rows<double, float>(nmx_ctx*, int, int, const int32_t*, const double*, int, float*, int);
 
 template <typename T>
 int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w, int32_t* ipiv, int32_t* info,

@@ -170,6 +170,20 @@ def lu_factor(a, block: int = 256, base: int = 32) -> LU:
     return LU(work, piv[:n], sign, int(info.item()))
 
 
+def laswp(b, pivots) -> torch.Tensor:
+    """laswp / laswp_matrix (numax/linalg/panel.mojo:1824-1852, :2279-2313):
+    apply the interchanges ``pivots`` (LAPACK order, 0-based) to the rows of a
+    float64 vector or n x k matrix ``b``; returns the permuted copy."""
+    x = _device_tensor(b, torch.float64).clone()
+    n = x.shape[0]
+    nrhs = 1 if x.dim() == 1 else x.shape[1]
+    piv = _device_tensor(pivots, torch.int32)
+    if piv.numel() != n:
+        raise NumaxError(f"laswp: {piv.numel()} pivots for {n} rows")
+    _lib.check(_lib.load().nmx_laswp_f64(_ctx(x).handle, n, nrhs, _ptr(x), max(nrhs, 1), _ptr(piv)), "laswp")
+    return x
+
+
 def solve(a, b, block: int = 256) -> torch.Tensor:
     """``lu_factor`` then the solve at the same device (basic.mojo:55-146)."""
     return lu_factor(a, block=block).solve(b)

@@ -53,6 +53,19 @@ def test_solve_triangular_matches_oracle(n, upper, unit, trans):
         np.testing.assert_allclose(x[:, c], xo, atol=1e-10, rtol=1e-10)
 
 
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 3), (300, 1), (4097, 16), (16384, 1), (16385, 2), (20000, 1)])
+def test_laswp_matches_oracle(n, nrhs):
+    # n <= 16384 forms the permutation in LDS, larger n in device memory
+    rng = np.random.default_rng(n + nrhs)
+    rows = np.arange(n)
+    ipiv = (rows + (rng.random(n) * (n - rows))).astype(np.int32)  # ipiv[j] in [j, n)
+    keep = rng.random(n) < 0.3
+    ipiv[keep] = rows[keep]  # steps without an interchange
+    b = rng.standard_normal((n, nrhs)) if nrhs > 1 else rng.standard_normal(n)
+    out = _np(nl.laswp(b, ipiv))
+    assert np.array_equal(out, oracle.laswp(b, ipiv))
+
+
 def _lu_parity(a, b, block=256, base=32):
     lu = nl.lu_factor(a, block=block, base=base)
     olu, opiv, oinfo, osign = oracle.getrf(a, block=32, base=16)
