@@ -103,6 +103,99 @@ def cpu_baseline(n_sample: int, threads: int) -> dict:
             "gflops": rate / 1e9, "finite": bool(np.isfinite(x).all())}
 
 
+def c4_batch(count: int, seed: int, torch):
+    """20,000-style uneven batch: n ~ U{8..256}; even problems LU with
+    A ~ U[-1,1] + 0.1 n I, odd problems Cholesky with A = G G^T / n + I;
+    b ~ N(0,1).  Generated on the GPU size by size and packed back to back."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = rng.integers(8, 257, size=count).astype(np.int32)
+    kind = (np.arange(count) % 2).astype(np.uint8)
+    offa = np.zeros(count, dtype=np.int64)
+    offb = np.zeros(count, dtype=np.int64)
+    offa[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    offb[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    a = torch.empty(int((n.astype(np.int64) ** 2).sum()), dtype=torch.float64, device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    for size in np.unique(n):
+        size = int(size)
+        for kd in (0, 1):
+            idx = np.flatnonzero((n == size) & (kind == kd))
+            if idx.size == 0:
+                continue
+            eye = torch.eye(size, dtype=torch.float64, device="cuda")
+            if kd == 0:
+                m = torch.rand((idx.size, size, size), dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+                m += 0.1 * size * eye
+            else:
+                g = torch.randn((idx.size, size, size), dtype=torch.float64, device="cuda", generator=gen)
+                m = torch.bmm(g, g.transpose(1, 2).contiguous()) / size + eye
+            dst = torch.as_tensor(offa[idx], device="cuda")[:, None] + torch.arange(size * size, device="cuda")
+            a[dst.reshape(-1)] = m.reshape(-1)
+    b = torch.randn(int(n.sum()), dtype=torch.float64, device="cuda", generator=gen)
+    return a, b, n, kind, offa, offb
+
+
+def batched_leg(args, torch) -> dict:
+    """Times the uneven batch (one nmx_batched_gesv per step) and reports its
+    HBM roofline: algorithmic bytes = sum(8 n^2 + 16 n) (A and b read once,
+    x written once, FP64)."""
+    import ctypes
+
+    import numpy as np
+
+    from numax_amd import _lib
+    from numax_amd import linalg as nl
+    a, b, n, kind, offa, offb = c4_batch(args.batch_count, 4, torch)
+    lib = _lib.load()
+    ctx = nl._ctx(a)
+    N = torch.as_tensor(n, device="cuda")
+    K = torch.as_tensor(kind, device="cuda")
+    OA = torch.as_tensor(offa, device="cuda")
+    OB = torch.as_tensor(offb, device="cuda")
+    X = torch.zeros_like(b)
+    info = torch.zeros(len(n), dtype=torch.int32, device="cuda")
+    iters = torch.zeros_like(info)
+    berr = torch.zeros(len(n), dtype=torch.float64, device="cuda")
+    cfg = _lib.BatchCfg()
+    lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def step():
+        _lib.check(lib.nmx_batched_gesv(ctx.handle, len(n), P(N), P(OA), P(OB), P(K), P(a), a.numel(), P(b),
+                                        b.numel(), P(X), P(info), P(iters), P(berr), ctypes.byref(cfg)),
+                   "nmx_batched_gesv")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    prof = profile_step(ctx, step)
+    nn = n.astype(np.float64)
+    bytes_alg = float((8 * nn ** 2 + 16 * nn).sum())
+    flops = float(np.where(kind == 0, 2 * nn ** 3 / 3, nn ** 3 / 3).sum())
+    be = berr.cpu().numpy()
+    inf = info.cpu().numpy()
+    it = iters.cpu().numpy()
+    rec = prof.get("batched_solve", {"ms": dt * 1e3, "launches": 1})
+    kern_s = rec["ms"] * 1e-3
+    out = {"workload": f"{len(n)} uneven systems, n ~ U{{8..256}}, half LU (U[-1,1] + 0.1 n I), "
+                       f"half Cholesky (G G^T / n + I), FP32 factor + FP64 refinement to {cfg.tol:g}",
+           "ms_per_step": dt * 1e3, "solves_per_s": len(n) / dt, "gflops": flops / dt / 1e9,
+           "roofline": {"bound": "hbm", "achieved": bytes_alg / kern_s / 1e9, "peak": PEAK["hbm_GBs"],
+                        "unit": "GB/s", "frac": bytes_alg / kern_s / 1e9 / PEAK["hbm_GBs"], "traffic": None,
+                        "bytes_per_step": bytes_alg, "kernel_ms": rec["ms"]},
+           "max_backward_error": float(np.nanmax(be)), "max_iterations": int(it.max()),
+           "failed": int((inf != 0).sum()), "kernel_profile": prof}
+    log("batched: " + json.dumps({k: v for k, v in out.items() if k != "kernel_profile"}))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +210,7 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=20000, help="systems in the uneven batch (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's path runs on a single MI355X "
@@ -230,6 +324,8 @@ def main() -> int:
     roof_plain = roofline_of(prof_plain, "plain-fp64")
     roof_plain["share_of_step"] = prof_plain[roof_plain["kernel"]]["ms"] / (dt_plain * 1e3)
 
+    batched = batched_leg(args, torch) if args.batch_count > 0 else None
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
@@ -265,6 +361,7 @@ def main() -> int:
         "mp_by_kappa": results,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_uneven": batched,
         "kernel_profile": prof_mp,
     }
     print(json.dumps(out), flush=True)

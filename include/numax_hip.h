@@ -53,7 +53,7 @@ int nmx_version(void);
 /* Optional per-kernel-class event timing on the context's stream (no
  * reference counterpart: it serves the roofline report).  Turning it on or
  * off clears the totals; nmx_ctx_profile synchronizes and returns the totals
- * of class `kind` (0 .. 9, names from nmx_kernel_name). */
+ * of class `kind` (0 .. 12, names from nmx_kernel_name). */
 int nmx_ctx_set_profiling(nmx_ctx* ctx, int on);
 int nmx_ctx_profile(nmx_ctx* ctx, int kind, int64_t* launches, double* ms, double* flops, double* bytes);
 const char* nmx_kernel_name(int kind);
@@ -147,6 +147,54 @@ int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b
  * pivot. */
 int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, int precision, int block, uint64_t seed,
                         int stage, int pivot, float* F, float* diags, int32_t* info, int32_t* ipiv);
+
+/* ---- batched uneven-size solves --------------------------------------------
+ * map_blocks over the register-tier solvers (numax/core/functional.mojo:691-834
+ * with numax/linalg/_array/lu.mojo:165-216 lu_factor + PivotedLU.solve, and
+ * numax/linalg/_array/cholesky.mojo:26-101 cholesky + cholesky_solve), past
+ * the reference's one-n-per-launch, n <= 64 limit: `count` independent
+ * systems A_p x_p = b_p of order n[p] in 1..256, each A_p row-major n x n at
+ * element offset offA[p] of A (a_elems elements in all), b_p and x_p at
+ * offset offB[p] of B and X; kind[p] 0 = LU with partial pivoting, 1 =
+ * Cholesky (lower triangle read); B and X hold b_elems elements.  Every
+ * array is device memory.  The
+ * factorization is FP32, the residual and the solution FP64, refined until
+ * the backward error ||b - A x||_inf / (||A||_inf ||x||_inf) <= tol.
+ * Per problem: info (first zero / non-positive pivot, 1-based; -1 for a
+ * problem outside n 1..256 or kind 0/1, which is skipped), iters (refinement
+ * steps) and berr (final backward error).  A problem whose matrix or
+ * vectors would fall outside a_elems / b_elems is skipped the same way. */
+typedef struct nmx_batch_cfg {
+  int max_iter;         /* refinement steps at most (default 10) */
+  double tol;           /* target backward error (default 1e-15) */
+  int teams_per_cu[3];  /* resident teams per CU for n > 128, 64 < n <= 128, n <= 64 */
+} nmx_batch_cfg;
+
+void nmx_batch_cfg_default(nmx_batch_cfg* cfg);
+int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                     const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems, double* X,
+                     int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg);
+
+/* ---- batched small symmetric eigensolver -----------------------------------
+ * The register-tier eigh (numax/linalg/_array/eigen.mojo:316-392) that
+ * map_blocks runs per matrix: cyclic Jacobi with numax's rotation
+ * (:34-75) and a fixed number of sweeps (numax's default is 12), FP32.
+ * `count` symmetric matrices of order n[p] in 1..64, row-major at element
+ * offset offA[p] of A; eigenvalues (unsorted, as numax) to W at offset
+ * offW[p], eigenvectors (columns, nullable V) at offA[p] of V.  The pairs
+ * are rotated in a parallel (odd-even) order, not numax's row-cyclic one:
+ * the converged values agree to rounding, not bit for bit.  A and V hold
+ * a_elems elements, W w_elems; problems with n outside 1..64, or whose
+ * matrix or eigenvalues would fall outside those sizes, are skipped. */
+int nmx_batched_syevj_f32(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offW,
+                          const float* A, int64_t a_elems, float* W, int64_t w_elems, float* V, int sweeps);
+
+/* ---- batched tall-skinny least squares ---------------------------------------
+ * lstsq(a, b) with method "qr" (numax/linalg/qr.mojo:746-821: Householder
+ * QR with LAPACK dlarfg reflectors, Q^T b, back substitution with R) for
+ * `count` problems A_p (m x n row-major, contiguous, 1 <= n <= 32, m >= n),
+ * b_p (m) and x_p (n), FP32, packed back to back. */
+int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const float* A, const float* B, float* X);
 
 /* ---- synthetic inputs ------------------------------------------------------
  * Philox4x32-10 stream (the same words as numax_amd.random): kind 0 fills

@@ -52,6 +52,12 @@ SIGNATURES = {
     "nmx_gemm_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_flt, _c_vp, _c_int, _c_vp,
                               _c_int, _c_flt, _c_vp, _c_int]),
     "nmx_fill_random_f64": (_c_int, [_c_vp, _c_int, _c_u64, _c_i64, _c_i64, _c_dbl, _c_dbl, _c_vp]),
+    "nmx_batch_cfg_default": (None, [_c_vp]),
+    "nmx_batched_syevj_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
+                                       _c_int]),
+    "nmx_batched_gels_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp]),
+    "nmx_batched_gesv": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
+                                  _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_ir_cfg_default": (None, [_c_vp]),
     "nmx_gesv_mpir": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_mp_factor_debug": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_int, _c_int, _c_u64, _c_int, _c_int,
@@ -70,6 +76,11 @@ class IrReport(ctypes.Structure):
     _fields_ = [("iterations", _c_int), ("gmres_iterations", _c_int), ("converged", _c_int), ("info", _c_int),
                 ("backward_error", _c_dbl), ("factor_ms", _c_dbl), ("refine_ms", _c_dbl),
                 ("history", _c_dbl * 32)]
+
+class BatchCfg(ctypes.Structure):
+    """``nmx_batch_cfg`` (include/numax_hip.h)."""
+    _fields_ = [("max_iter", _c_int), ("tol", _c_dbl), ("teams_per_cu", _c_int * 3)]
+
 
 _lib = None
 _lock = threading.Lock()
@@ -138,7 +149,7 @@ class Context:
         """Per kernel class: launches, milliseconds, algorithmic flops and bytes."""
         lib = load()
         out = {}
-        for kind in range(10):
+        for kind in range(13):
             n = _c_i64()
             ms, fl, by = _c_dbl(), _c_dbl(), _c_dbl()
             check(lib.nmx_ctx_profile(self.handle, kind, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl),

@@ -45,6 +45,9 @@ enum NmxKernel {
   K_RESIDUAL,     // FP64 residual b - A x
   K_MP_TRSV,      // low-precision triangular solves inside refinement
   K_OTHER,
+  K_BATCH,        // batched uneven-size solves (all size classes of one call)
+  K_EIGH,         // batched Jacobi eigensolver
+  K_TSQR,         // batched tall-skinny least squares
   K_COUNT
 };
 

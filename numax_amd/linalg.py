@@ -296,3 +296,105 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
         reports.append(info)
     x = xs[0] if B.dim() == 1 else torch.stack(xs, dim=1)
     return x, (reports[0] if B.dim() == 1 else reports)
+
+
+# ---- batched uneven-size solves ---------------------------------------------
+class BatchResult:
+    """Per-problem results of :func:`batched_solve`: the FP64 solutions
+    (packed like the right-hand sides), ``info`` (first zero / non-positive
+    pivot, 1-based; -1 for a problem that was not taken), refinement steps and
+    final backward error."""
+
+    def __init__(self, x, info, iters, berr, offsets_b, n):
+        self.x, self.info, self.iters, self.berr = x, info, iters, berr
+        self._offb, self._n = offsets_b, n
+
+    def solution(self, p: int) -> torch.Tensor:
+        o = int(self._offb[p])
+        return self.x[o:o + int(self._n[p])]
+
+
+def pack_systems(mats, rhss):
+    """Pack square matrices and vectors of uneven sizes into the flat layout
+    of :func:`batched_solve` (row-major, back to back)."""
+    n = np.array([np.asarray(m).shape[0] for m in mats], dtype=np.int32)
+    offa = np.zeros(len(mats), dtype=np.int64)
+    offb = np.zeros(len(mats), dtype=np.int64)
+    if len(mats):
+        offa[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+        offb[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    a = np.concatenate([np.asarray(m, dtype=np.float64).ravel() for m in mats]) if len(mats) else np.zeros(1)
+    b = np.concatenate([np.asarray(r, dtype=np.float64).ravel() for r in rhss]) if len(mats) else np.zeros(1)
+    return a, b, n, offa, offb
+
+
+def batched_solve(a, b, n, kind, offsets_a, offsets_b, tol: float = 1e-15, max_iter: int = 10,
+                  teams_per_cu=None) -> BatchResult:
+    """Solve ``len(n)`` independent systems of order 1..256 in one call: the
+    register-tier ``lu_factor`` + ``PivotedLU.solve`` (kind 0) and
+    ``cholesky`` + ``cholesky_solve`` (kind 1) of numax
+    (numax/linalg/_array/lu.mojo:165-216, numax/linalg/_array/cholesky.mojo:26-101)
+    that ``map_blocks`` runs one problem per thread
+    (numax/core/functional.mojo:691-834), past its n <= 64, one-n-per-launch
+    limit.  FP32 factorization, FP64 residual and refinement to backward error
+    ``tol``.  ``a``, ``b``: flat float64 arrays; ``offsets_*``: element offsets
+    of each problem's matrix / vector."""
+    A = _device_tensor(a, torch.float64)
+    B = _device_tensor(b, torch.float64)
+    count = len(n)
+    dev = A.device
+    N = _device_tensor(np.asarray(n, dtype=np.int32))
+    K = _device_tensor(np.asarray(kind, dtype=np.uint8))
+    OA = _device_tensor(np.asarray(offsets_a, dtype=np.int64))
+    OB = _device_tensor(np.asarray(offsets_b, dtype=np.int64))
+    X = torch.zeros_like(B)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    iters = torch.zeros(count, dtype=torch.int32, device=dev)
+    berr = torch.zeros(count, dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    cfg = _lib.BatchCfg()
+    lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+    cfg.tol = float(tol)
+    cfg.max_iter = int(max_iter)
+    if teams_per_cu is not None:
+        for i, v in enumerate(teams_per_cu):
+            cfg.teams_per_cu[i] = int(v)
+    _lib.check(lib.nmx_batched_gesv(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(OB), _ptr(K), _ptr(A), A.numel(),
+                                    _ptr(B), B.numel(), _ptr(X), _ptr(info), _ptr(iters), _ptr(berr),
+                                    ctypes.byref(cfg)), "batched_solve")
+    return BatchResult(X, info, iters, berr, np.asarray(offsets_b), np.asarray(n))
+
+
+# ---- batched small-matrix eigensolver and tall-skinny least squares --------------
+def eigh_batched(a, n, offsets_a, offsets_w, sweeps: int = 12, vectors: bool = True):
+    """numax's register-tier ``eigh`` (numax/linalg/_array/eigen.mojo:316-392)
+    over a batch of symmetric float32 matrices of order 1..64 packed back to
+    back (``offsets_a`` / ``offsets_w``: element offsets of each matrix and of
+    its eigenvalues).  Returns ``(w, v)``: unsorted eigenvalues and the
+    eigenvectors as columns (``v`` None unless ``vectors``)."""
+    A = _device_tensor(a, torch.float32)
+    count = len(n)
+    N = _device_tensor(np.asarray(n, dtype=np.int32))
+    OA = _device_tensor(np.asarray(offsets_a, dtype=np.int64))
+    OW = _device_tensor(np.asarray(offsets_w, dtype=np.int64))
+    w_len = int(np.max(np.asarray(offsets_w) + np.asarray(n))) if count else 1
+    W = torch.zeros(max(w_len, 1), dtype=torch.float32, device=A.device)
+    V = torch.zeros_like(A) if vectors else None
+    _lib.check(_lib.load().nmx_batched_syevj_f32(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(OW), _ptr(A),
+                                                 A.numel(), _ptr(W), W.numel(), _ptr(V) if vectors else None,
+                                                 int(sweeps)), "eigh")
+    return W, V
+
+
+def lstsq_batched(a, b) -> torch.Tensor:
+    """numax's ``lstsq`` (numax/linalg/qr.mojo:746-821, method "qr") for a
+    batch of float32 problems: ``a`` (count, m, n) with 1 <= n <= 32 and m >= n,
+    ``b`` (count, m); returns x (count, n)."""
+    A = _device_tensor(a, torch.float32)
+    B = _device_tensor(b, torch.float32)
+    if A.dim() != 3 or B.dim() != 2 or B.shape[0] != A.shape[0] or B.shape[1] != A.shape[1]:
+        raise NumaxError(f"lstsq: shapes {tuple(A.shape)} and {tuple(B.shape)} do not match (count, m, n) / (count, m)")
+    count, m, n = A.shape
+    X = torch.zeros((count, n), dtype=torch.float32, device=A.device)
+    _lib.check(_lib.load().nmx_batched_gels_f32(_ctx(A).handle, count, m, n, _ptr(A), _ptr(B), _ptr(X)), "lstsq")
+    return X

@@ -1,0 +1,125 @@
+"""GPU parity of the batched uneven-size solver (nmx_batched_gesv) against the
+oracle's register-tier algorithms (numax/linalg/_array/lu.mojo:165-216,
+numax/linalg/_array/cholesky.mojo:26-101) in float64.
+
+Bar: the GPU factors in FP32 and refines in FP64 until the backward error is
+<= 1e-15 (the default target), so every problem must report <= 1e-15 and
+measure <= 2e-15 with the oracle's residual, and its solution must agree with the FP64
+oracle solve to within cond(A) * 1e-15 (asserted as 1e-12 relative here on
+well-conditioned batches).  info / skip semantics are checked exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from numax_amd import linalg as nl
+from tests.golden import kat
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(sizes, seed):
+    """Half LU (A ~ U[-1,1] + 0.1 n I), half Cholesky (A = G G^T / n + I)."""
+    rng = np.random.default_rng(seed)
+    mats, rhss, kinds = [], [], []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            a = rng.uniform(-1.0, 1.0, (n, n)) + 0.1 * n * np.eye(n)
+            kinds.append(0)
+        else:
+            g = rng.standard_normal((n, n))
+            a = g @ g.T / n + np.eye(n)
+            kinds.append(1)
+        mats.append(a)
+        rhss.append(rng.standard_normal(n))
+    return mats, rhss, np.array(kinds, dtype=np.uint8)
+
+
+def _oracle_x(a, b, kind):
+    if kind == 0:
+        lu, perm, _ = oracle.array_lu_factor(a)
+        return oracle.array_lu_solve(lu, perm, b)
+    return oracle.array_cholesky_solve(oracle.array_cholesky(a), b)
+
+
+def _run(mats, rhss, kinds, **kw):
+    a, b, n, oa, ob = nl.pack_systems(mats, rhss)
+    res = nl.batched_solve(a, b, n, kinds, oa, ob, **kw)
+    return res, res.info.cpu().numpy(), res.iters.cpu().numpy(), res.berr.cpu().numpy()
+
+
+def _check(mats, rhss, kinds, res, info, berr):
+    x_all = res.x.cpu().numpy()
+    for p, (a, b, k) in enumerate(zip(mats, rhss, kinds)):
+        o = int(res._offb[p])
+        x = x_all[o:o + len(b)]
+        assert info[p] == 0, f"problem {p} (n={len(b)}, kind={k}) info {info[p]}"
+        assert berr[p] <= 1e-15, f"problem {p}: backward error {berr[p]}"  # the refinement target
+        assert oracle.backward_error(a, x, b) <= 2e-15, f"problem {p}: oracle-measured backward error"
+        xo = _oracle_x(a, b, k)
+        assert np.max(np.abs(x - xo)) <= 1e-12 * max(1.0, np.max(np.abs(xo))), f"problem {p} (n={len(b)}) x"
+
+
+@pytest.mark.parametrize("sizes", [
+    [1, 2, 3, 7, 8, 16, 31, 32, 33, 63, 64],          # one wavefront per problem
+    [65, 66, 100, 127, 128],                          # four wavefronts
+    [129, 130, 200, 255, 256],                        # sixteen wavefronts
+])
+def test_batched_size_classes_match_oracle(sizes):
+    mats, rhss, kinds = _systems(sizes + sizes[::-1], seed=len(sizes) * 7 + sizes[0])
+    res, info, iters, berr = _run(mats, rhss, kinds)
+    _check(mats, rhss, kinds, res, info, berr)
+
+
+def test_batched_mixed_uneven_batch():
+    rng = np.random.default_rng(2024)
+    sizes = [int(s) for s in rng.integers(8, 257, size=300)]
+    mats, rhss, kinds = _systems(sizes, seed=11)
+    res, info, iters, berr = _run(mats, rhss, kinds)
+    _check(mats, rhss, kinds, res, info, berr)
+    assert iters.max() <= 10
+
+
+def test_batched_map_blocks_spd_batch():
+    # tests/core/test_map_blocks.mojo:36-66: 1003 SPD 4x4 systems
+    a, b = kat.map_blocks_systems(1003, 4)
+    mats, rhss = list(a), list(b)
+    kinds = np.ones(len(mats), dtype=np.uint8)
+    res, info, iters, berr = _run(mats, rhss, kinds)
+    _check(mats, rhss, kinds, res, info, berr)
+
+
+def test_batched_known_answers():
+    mats = [kat.EXCHANGE, kat.TEXTBOOK_A, kat.NONSYM4, kat.PLU_A, kat.SPD5]
+    rhss = [kat.EXCHANGE_B, kat.TEXTBOOK_B, kat.NONSYM4_B, np.arange(1.0, 5.0), np.ones(5)]
+    kinds = np.array([0, 0, 0, 0, 1], dtype=np.uint8)
+    res, info, iters, berr = _run(mats, rhss, kinds)
+    assert np.allclose(res.solution(0).cpu().numpy(), kat.EXCHANGE_X, atol=1e-15)
+    assert np.allclose(res.solution(1).cpu().numpy(), kat.TEXTBOOK_X, atol=1e-14)
+    _check(mats, rhss, kinds, res, info, berr)
+
+
+def test_batched_info_and_skips():
+    mats = [np.zeros((3, 3)), kat.NOT_PD3, kat.LATE_NOT_PD, np.eye(2), np.eye(3)]
+    rhss = [np.ones(3), np.ones(3), np.ones(4), np.ones(2), np.ones(3)]
+    kinds = np.array([0, 1, 1, 2, 0], dtype=np.uint8)  # kind 2 is not a solver
+    a, b, n, oa, ob = nl.pack_systems(mats, rhss)
+    n = n.copy()
+    n[4] = 300  # past the 256 the kernels take
+    res = nl.batched_solve(a, b, n, kinds, oa, ob)
+    info = res.info.cpu().numpy()
+    assert info[0] == 1                       # exactly-zero first pivot, reported not raised
+    assert info[1] == kat.NOT_PD3_INFO        # 1-based, as the panel's info (test_panel.mojo:146-161)
+    assert info[2] == kat.LATE_NOT_PD_INDEX + 1  # the 0-based index cholesky's message names, + 1
+    assert info[3] == -1 and info[4] == -1    # skipped
+    assert np.isnan(res.berr.cpu().numpy()[3])
+
+
+def test_batched_rejects_out_of_range_offsets():
+    mats, rhss, kinds = _systems([5, 6], seed=3)
+    a, b, n, oa, ob = nl.pack_systems(mats, rhss)
+    oa = oa.copy()
+    oa[1] = len(a) - 10  # 6 x 6 would run past the end of A
+    res = nl.batched_solve(a, b, n, kinds, oa, ob)
+    info = res.info.cpu().numpy()
+    assert info[0] == 0 and info[1] == -1
