@@ -211,6 +211,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=20000, help="systems in the uneven batch (0: skip it)")
+    ap.add_argument("--matrix", default="haar", choices=["haar", "normal"],
+                    help="haar: U diag(sigma) V^T (the workload); normal: N(0,1) entries (no QR in the setup, for "
+                         "counter-collection runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's path runs on a single MI355X "
@@ -225,7 +228,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     log(f"generating A (n={n}, kappa={args.kappa:g}) on the GPU")
     t0 = time.perf_counter()
-    a = c3_matrix(n, args.kappa, torch)
+    a = c3_matrix(n, args.kappa, torch) if args.matrix == "haar" else R.device_normal(3, (n, n))
     b = R.device_normal(4, (n, nrhs)) if nrhs > 1 else R.device_normal(4, (n,))
     torch.cuda.synchronize()
     log(f"inputs ready in {time.perf_counter() - t0:.1f} s")

@@ -62,241 +62,295 @@ __device__ __forceinline__ void rotate_left(T (&v)[N]) {
   v[N - 1] = first;
 }
 
+// ---- wavefront reductions on DPP (no LDS traffic): every lane of a 16-lane
+// row gets the row's extreme, then four lanes are read into scalar registers.
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+  const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  const uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return max(max(a, b), max(c, d));
+}
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+  const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  const uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return min(min(a, b), min(c, d));
+}
+
+// Pivot keys: a candidate's magnitude as its (order-preserving) bit pattern
+// plus one, 0 for "no candidate".  Equal keys are equal magnitudes, so the
+// first maximum is the smallest row among the largest key.
+template <typename T>
+struct PivKey;
+template <>
+struct PivKey<float> {
+  typedef uint32_t K;
+  static __device__ __forceinline__ K of(float mag) { return __float_as_uint(mag) + 1u; }
+  static __device__ __forceinline__ K wave_max(K k) { return wave_umax(k); }
+  static constexpr int WORDS = 1;
+  static __device__ __forceinline__ uint32_t word(K k, int i) { return k; }
+  static __device__ __forceinline__ K from(const uint32_t* w) { return w[0]; }
+};
+template <>
+struct PivKey<double> {
+  typedef uint64_t K;
+  static __device__ __forceinline__ K of(double mag) { return (uint64_t)__double_as_longlong(mag) + 1ull; }
+  static __device__ __forceinline__ K wave_max(K k) {
+    const uint32_t hi = wave_umax((uint32_t)(k >> 32));
+    const uint32_t lo = wave_umax((uint32_t)(k >> 32) == hi ? (uint32_t)k : 0u);
+    return ((uint64_t)hi << 32) | lo;
+  }
+  static constexpr int WORDS = 2;
+  static __device__ __forceinline__ uint32_t word(K k, int i) { return i ? (uint32_t)(k >> 32) : (uint32_t)k; }
+  static __device__ __forceinline__ K from(const uint32_t* w) { return ((uint64_t)w[1] << 32) | w[0]; }
+};
+
 // ---------------------------------------------------------------------------
 // Cooperative leaf of the recursive LU panel.
 //
 // Factors columns c0..c0+w (w <= 32) over rows r0..n with partial pivoting,
 // exactly getrf_panel's arithmetic (panel.mojo:289-431): first-maximum pivot
 // (ties to the lower row; the diagonal row counts as magnitude >= 0, so an
-// all-zero or all-NaN column keeps its diagonal), interchange, an exact-zero
-// pivot divided as 1e-30 with `info` recording the first, true division for
-// the multipliers and the rank-1 update of the panel's own columns.  The
-// leaf's interchanges are then replayed over every other column of the
-// matrix right after (laswp_gather_kernel), which leaves the matrix numax's
-// full-width swaps leave (panel.mojo:404-411).
+// all-zero or all-NaN column keeps its diagonal; NaN never wins),
+// interchange, an exact-zero pivot divided as 1e-30 with `info` recording the
+// first, true division for the multipliers and the rank-1 update of the
+// panel's own columns.  The leaf's interchanges are then replayed over every
+// other column of the matrix right after (laswp_gather_kernel), which leaves
+// the matrix numax's full-width swaps leave (panel.mojo:404-411).
 //
-// Workgroup p of `parts` owns rows r0 + p*1024 ..., one row per thread in
-// registers.  With parts > 1 the grid is 8*parts workgroups of which every
-// eighth participates, so on MI355X (workgroup b runs on XCD b % 8) all
-// participants share one XCD and its L2; correctness does not depend on that
-// placement, every exchange being an agent-scope atomic.
+// Workgroup p of `parts` owns rows r0 + 1024 p + t + 512 i (i = 0, 1) of
+// its 512 threads, each row's 32 columns in registers.  The column loop stays
+// rolled (unrolled it overflows the instruction cache); the rows are rotated
+// left by one after each column so register indices stay static: at step j,
+// v[.][0] is column j and v[.][1 .. w-j) are the columns still to update.
+// Per column, in one workgroup: DPP reductions of the pivot key and row, each
+// wavefront's winning row staged in LDS, one barrier.  Across workgroups
+// (parts > 1, grid 8 * parts of which every eighth takes part, so all
+// participants share one XCD and its L2): each part publishes its winner,
+// its key and (part 0) the diagonal row as epoch-tagged 64-bit words, and
+// every thread of every part spins on the words it stages -- one round trip,
+// no fences.  Slots alternate by epoch parity.
 // ---------------------------------------------------------------------------
+constexpr int kLeafRows = 2;                    // rows per thread
+constexpr int kLeafT = kLeafThreads / kLeafRows;  // 512 threads
+constexpr int kLeafWaves = kLeafT / 64;
+
 template <typename T>
-__global__ void __launch_bounds__(1024) getf2_leaf(T* __restrict__ A, int lda, int n, int r0, int c0, int w,
-                                                   int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
-                                                   uint64_t* __restrict__ mb, uint32_t epoch0, int parts, int* err) {
+__global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda, int n, int r0, int c0, int w,
+                                                     int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                                     uint64_t* __restrict__ mb, uint32_t epoch0, int parts, int* err) {
   constexpr int W = kLeafW;
   constexpr int VW = MbLayout<T>::VW;
   constexpr int STRIDE = MbLayout<T>::STRIDE;
+  typedef PivKey<T> PK;
+  typedef typename PK::K Key;
   const bool multi = parts > 1;
   if (multi && (blockIdx.x % kXcds) != 0) return;
   const int p = multi ? (int)(blockIdx.x / kXcds) : 0;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
 
-  __shared__ T red_mag[16];
-  __shared__ int red_row[16];
-  __shared__ __attribute__((aligned(16))) uint32_t cand_bits[kMaxParts][W * VW];
-  __shared__ __attribute__((aligned(16))) uint32_t mag_bits[kMaxParts][VW > 1 ? VW : 2];
-  __shared__ int cand_row[kMaxParts];
-  __shared__ __attribute__((aligned(16))) uint32_t diag_bits[W * VW];
-  __shared__ T u_s[W];
-  __shared__ T d_s[W];
-  __shared__ int win_row, win_slot, aborted, info_set;
+  // per-column staging, double-buffered by column parity
+  __shared__ __attribute__((aligned(16))) T slot_v[2][kLeafWaves][W];  // each wavefront's winning row
+  __shared__ Key slot_k[2][kLeafWaves];
+  __shared__ int slot_r[2][kLeafWaves];
+  __shared__ __attribute__((aligned(16))) T diag_v[2][W];               // the diagonal row
+  // staged mailbox words, per part [values | key | row] (PSTRIDE even, so
+  // every part's values stay 8-byte aligned), then the diagonal row
+  constexpr int PSTRIDE = (W * VW + VW + 1 + 1) & ~1;
+  __shared__ __attribute__((aligned(16))) uint32_t stage[2][kMaxParts * PSTRIDE + W * VW];
+  __shared__ int aborted, info_set;
 
   if (t == 0) {
     aborted = 0;
     info_set = (*info != 0);
   }
 
-  const int row = r0 + p * kLeafThreads + t;
-  const bool own = row < n;
-  T v[W];
+  const int row0 = r0 + p * kLeafThreads + t;  // row i = row0 + 512 i
+  T v[kLeafRows][W];
   {
-    // descriptor at this workgroup's first row: offsets stay below 1024 rows
+    // one descriptor at this workgroup's first row: offsets stay below 1024 rows
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + (int64_t)(r0 + p * kLeafThreads) * lda + c0);
-    const uint32_t rel = (uint32_t)t * (uint32_t)lda;
 #pragma unroll
-    for (int c = 0; c < W; c++) v[c] = buf_load<T>(rs, (own & (c < w)) ? (rel + c) * (uint32_t)sizeof(T) : kBufOob);
+    for (int i = 0; i < kLeafRows; i++) {
+      const bool own = row0 + kLeafT * i < n;
+      const uint32_t rel = (uint32_t)(t + kLeafT * i) * (uint32_t)lda;
+#pragma unroll
+      for (int c = 0; c < W; c++)
+        v[i][c] = buf_load<T>(rs, (own & (c < w)) ? (rel + c) * (uint32_t)sizeof(T) : kBufOob);
+    }
   }
   __syncthreads();
 
-  // The column loop stays rolled (a fully unrolled one is ~230 KB of code,
-  // far beyond the instruction cache, and every column would refetch it).
-  // To keep register indices static, the row held in v[] is rotated left by
-  // one after each column: at step j, v[0] is column j, v[1 .. w-j) are the
-  // columns still to update, and the finished ones sit at the end.  After
-  // the last column the rotation is completed to a full turn.
   for (int j = 0; j < w; j++) {
-    {
-      const int gr = r0 + j;
-      const int gc = c0 + j;
-      const uint32_t epoch = epoch0 + (uint32_t)j + 1u;
-      uint64_t* par = mb + (size_t)(epoch & 1u) * MbLayout<T>::PARITY;
+    const int gr = r0 + j;
+    const int gc = c0 + j;
+    const int par = j & 1;
+    const uint32_t epoch = epoch0 + (uint32_t)j + 1u;
 
-      // 1. candidate: this thread's row, then the wavefront's, then the workgroup's
-      T bm = T(-1);
-      int br = INT_MAX;
-      if (own && row >= gr) {
-        T mag = fabs(v[0]);
+    // 1. this thread's candidate: largest key, smallest row
+    Key tk = 0;
+    int trow = INT_MAX;
+#pragma unroll
+    for (int i = 0; i < kLeafRows; i++) {
+      const int row = row0 + kLeafT * i;
+      if (row >= gr && row < n) {
+        T mag = fabs(v[i][0]);
         if (!(mag == mag)) mag = T(-1);           // NaN never wins ...
         if (row == gr && mag < T(0)) mag = T(0);  // ... the diagonal row starts at 0
-        bm = mag;
-        br = row;
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        T om = __shfl_xor(bm, off);
-        int orr = __shfl_xor(br, off);
-        if (better(om, orr, bm, br)) { bm = om; br = orr; }
-      }
-      if (lane == 0) { red_mag[wave] = bm; red_row[wave] = br; }
-      __syncthreads();
-      if (t < 64) {
-        bm = t < 16 ? red_mag[t] : T(-1);
-        br = t < 16 ? red_row[t] : INT_MAX;
-#pragma unroll
-        for (int off = 8; off > 0; off >>= 1) {
-          T om = __shfl_xor(bm, off);
-          int orr = __shfl_xor(br, off);
-          if (better(om, orr, bm, br)) { bm = om; br = orr; }
+        if (mag >= T(0)) {
+          const Key k = PK::of(mag);
+          if (k > tk) { tk = k; trow = row; }   // rows ascend with i: ties keep the first
         }
-        if (t == 0) { red_mag[0] = bm; red_row[0] = br; win_row = br; }
       }
-      __syncthreads();
-      const T wg_mag = red_mag[0];
-      const int wg_row = red_row[0];
-
-      if (!multi) {
-        if (own && row == wg_row) {
+    }
+    // 2. the wavefront's: largest key, then the smallest row holding it
+    const Key wk = PK::wave_max(tk);
+    const int wrow = (int)wave_umin(tk == wk ? (uint32_t)trow : 0xFFFFFFFFu);
+    // its row values (and the diagonal row's) into LDS
 #pragma unroll
-          for (int c = 0; c < W; c++) u_s[c] = v[c];
+    for (int i = 0; i < kLeafRows; i++) {
+      const int row = row0 + kLeafT * i;
+      if (wk != 0 && row == wrow) {
+#pragma unroll
+        for (int c = 0; c < W; c++) slot_v[par][wave][c] = v[i][c];
+      }
+      if (row == gr) {
+#pragma unroll
+        for (int c = 0; c < W; c++) diag_v[par][c] = v[i][c];
+      }
+    }
+    if (lane == 0) {
+      slot_k[par][wave] = wk;
+      slot_r[par][wave] = wrow;
+    }
+    __syncthreads();
+    // 3. the workgroup's winner (every wavefront works it out)
+    Key gk = 0;
+    int grow = INT_MAX, gw = 0;
+#pragma unroll
+    for (int q = 0; q < kLeafWaves; q++) {
+      const Key k = slot_k[par][q];
+      const int r = slot_r[par][q];
+      if (k > gk || (k == gk && k != 0 && r < grow)) { gk = k; grow = r; gw = q; }
+    }
+    const T* urow;
+    const T* drow;
+    if (!multi) {
+      urow = slot_v[par][gw];
+      drow = diag_v[par];
+    } else {
+      // 4. publish this part's winner (or "none") and, from part 0, the diagonal row
+      uint64_t* mbp = mb + (size_t)(epoch & 1u) * MbLayout<T>::PARITY;
+      uint64_t* myslot = mbp + (size_t)p * STRIDE;
+      if (t < W * VW) {
+        const uint32_t* src = (const uint32_t*)slot_v[par][gw];
+        mb_put(myslot + 1 + VW + t, gk != 0 ? src[t] : 0u, epoch);
+      } else if (t < W * VW + VW) {
+        mb_put(myslot + 1 + (t - W * VW), PK::word(gk, t - W * VW), epoch);
+      } else if (t == W * VW + VW) {
+        mb_put(myslot, (uint32_t)grow, epoch);
+      } else if (p == 0 && t >= 128 && t < 128 + W * VW) {
+        mb_put(mbp + (size_t)kMaxParts * STRIDE + (t - 128), ((const uint32_t*)diag_v[par])[t - 128], epoch);
+      }
+      // 5. stage every part's words (and the diagonal row), each validated by its epoch
+      const int words = parts * STRIDE + W * VW;
+      uint32_t* stg = stage[par];
+      for (int idx = t; idx < words; idx += kLeafT) {
+        const uint64_t* src = idx < parts * STRIDE ? mbp + idx : mbp + (size_t)kMaxParts * STRIDE + (idx - parts * STRIDE);
+        uint64_t wv;
+        long spins = 0;
+        for (;;) {
+          wv = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((uint32_t)wv == epoch) break;
+          if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
+          __builtin_amdgcn_s_sleep(1);
         }
-        if (own && row == gr) {
-#pragma unroll
-          for (int c = 0; c < W; c++) d_s[c] = v[c];
-        }
-        __syncthreads();
-      } else {
-        // 2. publish: the winning row (or a no-candidate marker) and the diagonal row
-        uint64_t* slot = par + (size_t)p * STRIDE;
-        if (wg_row != INT_MAX) {
-          if (own && row == wg_row) {
-#pragma unroll
-            for (int c = 0; c < W; c++) mb_put_value<T>(slot + 1 + VW + c * VW, v[c], epoch);
-            mb_put_value<T>(slot + 1, wg_mag, epoch);
-            mb_put(slot, (uint32_t)wg_row, epoch);  // header last: pollers watch it
-          }
+        int dst;
+        if (idx < parts * STRIDE) {
+          const int q = idx / STRIDE, f = idx % STRIDE;  // mailbox slot: [row | key | values]
+          dst = q * PSTRIDE + (f == 0 ? W * VW + VW : (f <= VW ? W * VW + f - 1 : f - 1 - VW));
         } else {
-          if (t < W * VW) mb_put(slot + 1 + VW + t, 0u, epoch);
-          if (t == 0) {
-            mb_put_value<T>(slot + 1, T(-1), epoch);
-            mb_put(slot, (uint32_t)INT_MAX, epoch);
-          }
+          dst = kMaxParts * PSTRIDE + (idx - parts * STRIDE);
         }
-        if (own && row == gr) {
-          uint64_t* dslot = par + (size_t)kMaxParts * STRIDE;
-#pragma unroll
-          for (int c = 0; c < W; c++) mb_put_value<T>(dslot + c * VW, v[c], epoch);
-        }
-        // 3. one poller per participant slot (with backoff), then one pass over
-        //    every participant's words, each still validated by its epoch
-        if (t < parts) {
-          long spins = 0;
-          while ((uint32_t)__hip_atomic_load(par + (size_t)t * STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-                 epoch) {
-            if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
-            __builtin_amdgcn_s_sleep(1);
-          }
-        }
-        __syncthreads();
-        const int words = parts * STRIDE + W * VW;
-        for (int idx = t; idx < words; idx += kLeafThreads) {
-          const uint64_t* src = idx < parts * STRIDE ? par + idx : par + (size_t)kMaxParts * STRIDE + (idx - parts * STRIDE);
-          uint64_t wv;
-          long spins = 0;
-          for (;;) {
-            wv = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((uint32_t)wv == epoch) break;
-            if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
-            __builtin_amdgcn_s_sleep(1);
-          }
-          const uint32_t payload = (uint32_t)(wv >> 32);
-          if (idx < parts * STRIDE) {
-            const int s2 = idx / STRIDE, f = idx % STRIDE;
-            if (f == 0) cand_row[s2] = (int)payload;
-            else if (f <= VW) mag_bits[s2][f - 1] = payload;
-            else cand_bits[s2][f - 1 - VW] = payload;
-          } else {
-            diag_bits[idx - parts * STRIDE] = payload;
-          }
-        }
-        __syncthreads();
-        if (t < 64) {
-          T cm = T(-1);
-          int cr = INT_MAX, cs = t < parts ? t : 0;
-          if (t < parts) {
-            cm = *(const T*)&mag_bits[t][0];
-            cr = cand_row[t];
-          }
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) {
-            T om = __shfl_xor(cm, off);
-            int orr = __shfl_xor(cr, off);
-            int os = __shfl_xor(cs, off);
-            if (better(om, orr, cm, cr)) { cm = om; cr = orr; cs = os; }
-          }
-          if (t == 0) { win_row = cr; win_slot = cs; }
-        }
-        __syncthreads();
-        if (t < W) {
-          u_s[t] = ((const T*)&cand_bits[win_slot][0])[t];
-          d_s[t] = ((const T*)&diag_bits[0])[t];
-        }
-        __syncthreads();
+        stg[dst] = (uint32_t)(wv >> 32);
       }
+      __syncthreads();
+      // 6. the global winner: largest key, then smallest row
+      Key bk = 0;
+      int brow = INT_MAX, bp = 0;
+      for (int q = 0; q < parts; q++) {
+        const uint32_t* sq = stg + q * PSTRIDE;
+        const Key k = PK::from(sq + W * VW);
+        const int r = (int)sq[W * VW + VW];
+        if (k > bk || (k == bk && k != 0 && r < brow)) { bk = k; brow = r; bp = q; }
+      }
+      gk = bk;
+      grow = brow;
+      urow = (const T*)(stg + bp * PSTRIDE);
+      drow = (const T*)(stg + kMaxParts * PSTRIDE);
+    }
 
-      const int wrow = (win_row == INT_MAX || aborted) ? gr : win_row;
-      // 4. interchange inside the leaf's columns, record the pivot
-      if (wrow != gr && own) {
+    const int wr = (gk == 0 || aborted) ? gr : grow;
+    if (wr == gr) urow = drow;
+    // 7. interchange inside the leaf's columns, record the pivot
+#pragma unroll
+    for (int i = 0; i < kLeafRows; i++) {
+      const int row = row0 + kLeafT * i;
+      if (wr != gr && row < n) {
         if (row == gr) {
 #pragma unroll
-          for (int c = 0; c < W; c++) v[c] = u_s[c];
-        } else if (row == wrow) {
+          for (int c = 0; c < W; c++) v[i][c] = urow[c];
+        } else if (row == wr) {
 #pragma unroll
-          for (int c = 0; c < W; c++) v[c] = d_s[c];
+          for (int c = 0; c < W; c++) v[i][c] = drow[c];
         }
       }
-      const T* urow = (wrow != gr) ? u_s : d_s;  // the pivot row, LDS broadcast reads
-      T pivot = urow[0];
-      if (t == 0 && p == 0) {
-        ipiv[gc] = wrow;
-        if (fabs(pivot) == T(0) && !info_set) {
-          *info = gc + 1;
-          info_set = 1;
-        }
+    }
+    T pivot = urow[0];
+    if (t == 0 && p == 0) {
+      ipiv[gc] = wr;
+      if (fabs(pivot) == T(0) && !info_set) {
+        *info = gc + 1;
+        info_set = 1;
       }
-      if (pivot == T(0)) pivot = T(kPivotFloor);
-      // 5. multipliers and the rank-1 update of the leaf's remaining columns
-      if (own && row > gr) {
-        const T l = v[0] / pivot;
-        v[0] = l;
-        const int rem = w - j;
+    }
+    if (pivot == T(0)) pivot = T(kPivotFloor);
+    // 8. multipliers and the rank-1 update of the leaf's remaining columns
+    const int rem = w - j;
 #pragma unroll
-        for (int c = 1; c < W; c++) v[c] = (c < rem) ? v[c] - l * urow[c] : v[c];
+    for (int i = 0; i < kLeafRows; i++) {
+      const int row = row0 + kLeafT * i;
+      if (row > gr && row < n) {
+        const T l = v[i][0] / pivot;
+        v[i][0] = l;
+#pragma unroll
+        for (int c = 1; c < W; c++) v[i][c] = (c < rem) ? v[i][c] - l * urow[c] : v[i][c];
       }
-      // u_s / d_s are next written after two barriers of the next column
-      rotate_left(v);
+      rotate_left(v[i]);
     }
   }
-  for (int j = w; j < W; j++) rotate_left(v);
+  for (int j = w; j < W; j++) {
+#pragma unroll
+    for (int i = 0; i < kLeafRows; i++) rotate_left(v[i]);
+  }
 
   if (aborted && t == 0) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
-  if (own) {
-    T* dst = A + (int64_t)row * lda + c0;
 #pragma unroll
-    for (int c = 0; c < W; c++)
-      if (c < w) dst[c] = v[c];
+  for (int i = 0; i < kLeafRows; i++) {
+    const int row = row0 + kLeafT * i;
+    if (row < n) {
+      T* dst = A + (int64_t)row * lda + c0;
+#pragma unroll
+      for (int c = 0; c < W; c++)
+        if (c < w) dst[c] = v[i][c];
+    }
   }
 }
 
@@ -788,7 +842,7 @@ int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w,
     // A participant that is not resident makes the others' bounded waits
     // expire, which the context reports as NMX_ERR_SYNC_TIMEOUT.
     const int grid = parts == 1 ? 1 : parts * kXcds;
-    getf2_leaf<T><<<grid, kLeafThreads, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
+    getf2_leaf<T><<<grid, kLeafT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
     NMX_HIP(ctx, hipGetLastError());
   }
   if (swap_others) return laswp_launch<T>(ctx, A, lda, 0, n, c0, c0 + w, ipiv, r0, r0 + w);
