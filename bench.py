@@ -211,6 +211,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=20000, help="systems in the uneven batch (0: skip it)")
+    ap.add_argument("--batched-only", action="store_true", help="time only the uneven batch (prints its object)")
     ap.add_argument("--matrix", default="haar", choices=["haar", "normal"],
                     help="haar: U diag(sigma) V^T (the workload); normal: N(0,1) entries (no QR in the setup, for "
                          "counter-collection runs)")
@@ -224,6 +225,9 @@ def main() -> int:
 
     from numax_amd import linalg as nl
     from numax_amd import random as R
+    if args.batched_only:
+        print(json.dumps(batched_leg(args, torch)), flush=True)
+        return 0
     n, nrhs = args.n, args.nrhs
     dev = torch.device("cuda:0")
     log(f"generating A (n={n}, kappa={args.kappa:g}) on the GPU")

@@ -5,26 +5,27 @@
 // `map_blocks` (numax/core/functional.mojo:691-834) over the register-tier
 // algorithms -- pivoted LU (numax/linalg/_array/lu.mojo:165-216) and
 // Cholesky (numax/linalg/_array/cholesky.mojo:26-101) -- with one compile-time
-// n per launch and n <= 64 (numax/linalg/common.mojo:30-35).  Here one launch
-// per size class takes every problem of that class from a queue sorted by
-// size (largest first): a team of G*G wavefronts owns one problem at a time,
-// G lanes per matrix row, each lane holding 64 consecutive columns of its row
-// in registers:
+// n per launch and n <= 64 (numax/linalg/common.mojo:30-35).  Here one
+// wavefront owns one problem at a time and factors it blocked, right-looking,
+// 32 columns per panel:
 //
-//   class 0: 128 < n <= 256, G = 4, 1024 threads
-//   class 1:  64 < n <= 128, G = 2,  256 threads
-//   class 2:       n <=  64, G = 1,   64 threads (one wavefront, no barrier
-//                                                 is ever waited on)
+//   - the panel lives in registers (lane l holds rows k + l + 64 q), its
+//     pivot search is a DPP key reduction, its pivot row is read lane-to-
+//     scalar, so a column costs no barrier and no LDS round trip but the
+//     interchange;
+//   - the panel's interchanges reach the other columns as one gather of the
+//     net row moves;
+//   - U12 = L11^-1 A12 (LU) or L21 = A21 L11^-T (Cholesky) is one
+//     triangular solve per lane;
+//   - the trailing update A22 -= L21 U12 (or the lower triangle of
+//     A22 -= L21 L21^T) is 32 x 32 tiles of v_mfma_f32_32x32x2f32.
 //
-// The factorization is right-looking, one column per step, with every
-// register index static: column k is read with a select tree on k, the
-// pivot row is broadcast through LDS with its columns <= k zeroed, so the
-// rank-1 update runs over all 64 registers and leaves column k's raw values
-// in place; the multipliers (raw / pivot) are formed once, by the same
-// division, when the factor is written out.  The finished factor goes to an
-// FP32 scratch copy (L2-resident while the team works on it); the solves
-// read it blockwise, 64 rows at a time; the residual b - A x is FP64 from
-// the original matrix.
+// The FP32 working copy is a scratch matrix per problem (L2-resident while
+// its wavefront works on it); the solves read it 64 rows at a time; the
+// residual b - A x is FP64 from the original matrix.  Problems are sorted by
+// kind and size class (rows per lane Q = 1..4) on the device and handed out
+// largest first from per-class queues: the kernels are persistent and need
+// no host round trip.
 #include <limits.h>
 #include <math.h>
 
@@ -33,12 +34,16 @@
 namespace {
 
 constexpr int kBatchMaxN = 256;
-constexpr int kSizeClasses = 3;
+constexpr int kSizeClasses = 4;             // n in (64 (c-1), 64 c]: Q = c rows per lane
 constexpr int kClasses = 2 * kSizeClasses;  // (LU, Cholesky) x size class
 constexpr int kBins = 2 * (kBatchMaxN + 1);  // histogram bin = kind * 257 + n
-constexpr float kFloorF = 1e-30f;  // numax/linalg/common.mojo:27-29
+constexpr int kNB = 32;                     // panel width
+constexpr int kWaves = 4;                   // independent wavefronts per workgroup
+constexpr float kFloorF = 1e-30f;           // numax/linalg/common.mojo:27-29
 
-__host__ __device__ constexpr int class_of(int n) { return n > 128 ? 0 : (n > 64 ? 1 : 2); }
+__host__ __device__ constexpr int class_of(int n) { return (n - 1) >> 6; }
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // guard_nonzero (numax/core/numeric.mojo:346-364): |x| raised to the floor,
 // sign kept, +floor at +0, NaN passes through.
@@ -46,13 +51,18 @@ __device__ __forceinline__ float guard_f(float x) {
   const float m = fabsf(x);
   return (m >= kFloorF || m != m) ? x : copysignf(kFloorF, x);
 }
-__device__ __forceinline__ double guard_d(double x) {
-  const double m = fabs(x);
-  return (m >= 1e-30 || m != m) ? x : copysign(1e-30, x);
-}
 
-__device__ __forceinline__ float readlane_f(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+// Orders a wavefront's LDS and global accesses around cross-lane hand-offs
+// (no other wavefront shares the data, so no barrier is needed or allowed:
+// the wavefronts of a workgroup work on different problems).
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+// max over the wavefront of a non-negative double, as its bit pattern
+__device__ __forceinline__ double wave_dmax(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint32_t hi = wave_umax((uint32_t)(b >> 32));
+  const uint32_t lo = wave_umax((uint32_t)(b >> 32) == hi ? (uint32_t)b : 0u);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 struct BatchArgs {
@@ -67,271 +77,387 @@ struct BatchArgs {
   int32_t* info;
   int32_t* iters;
   double* berr;
-  float* F;               // FP32 factor scratch, same offsets as A
-  const int32_t* order;   // problem indices, largest n first
-  const int32_t* cls;     // [0..3) class begin, [3..6) class count
-  int32_t* head;          // per-class queue heads
+  float* F;              // FP32 working copies, same offsets as A
+  const int32_t* order;  // problem indices, by class, largest n first
+  const int32_t* cls;    // [0..8) class begin, [8..16) class count
+  int32_t* head;         // per-class queue heads
   int max_iter;
   double tol;
 };
 
-template <int G>
-struct Team {
-  static constexpr int NMAX = 64 * G;
-  static constexpr int T = 64 * G * G;
-  static constexpr int NW = G * G;  // wavefronts
+struct WaveLds {
+  float xs[kBatchMaxN];          // FP32 solve vector
+  double x64[kBatchMaxN];        // solution
+  double r64[kBatchMaxN];        // residual
+  int perm[kBatchMaxN];          // original row now at row i (LU)
+  int ipiv[kBatchMaxN];          // interchanges, LAPACK order
+  float rowbuf[2][kNB];          // interchange staging
+  int mv_dst[2 * kNB], mv_src[2 * kNB];
 };
 
-template <int G>
-struct BatchShared {
-  float P[Team<G>::NMAX];       // pivot row (LU), full
-  float U[Team<G>::NMAX];       // pivot row with finished positions zeroed
-  float K[Team<G>::NMAX];       // old row k (LU interchange)
-  float col[2][Team<G>::NMAX];  // column k below the diagonal (Cholesky), double-buffered
-  float dpiv[Team<G>::NMAX];    // guarded pivots (LU) / ljj (Cholesky)
-  float pcur, kcur;
-  float cmag[16];
-  int crow[16];
-  int perm[Team<G>::NMAX];
-  float xs[Team<G>::NMAX];      // FP32 solve vector
-  double x64[Team<G>::NMAX];
-  double r64[Team<G>::NMAX];
-  unsigned long long norms[4];  // max|r|, max|x|, ||A||_inf (bits of non-negative doubles)
-  int slot, info;
-};
-
-// v[i] -= s * w[i] over the 64 registers, w in LDS; eight at a time with a
-// scheduling barrier between groups, so at most eight LDS values are live
-// (a fully hoisted 64-value prefetch would double the register footprint).
-__device__ __forceinline__ void axpy_lds(float (&v)[64], float s, const float* w) {
+// ---- panel: pivoted LU of F[k:n, k:k+nb] in registers ------------------------
+// PivotedLU's arithmetic (_array/lu.mojo:165-216): first maximum of |a_ik|
+// with the diagonal as the incumbent (a NaN incumbent is kept, a NaN never
+// displaces one), full-row interchange, pivot guard_nonzero(a_kk),
+// multipliers by true division.  Column J of the panel is a template
+// instance, so every register index is static however large the unrolled
+// panel gets (a partially unrolled loop would index pv dynamically and move
+// it to scratch memory).
+template <int Q, int J>
+__device__ __forceinline__ void lu_column(float (&pv)[Q][kNB], int n, int k, WaveLds& S, int lane, int& info) {
+  const int kj = k + J;
+  uint32_t key = 0;
+  int crow = INT_MAX;
 #pragma unroll
-  for (int c = 0; c < 64; c += 8) {
-    const float4 a = *(const float4*)(w + c);
-    const float4 b = *(const float4*)(w + c + 4);
-    v[c + 0] -= s * a.x;
-    v[c + 1] -= s * a.y;
-    v[c + 2] -= s * a.z;
-    v[c + 3] -= s * a.w;
-    v[c + 4] -= s * b.x;
-    v[c + 5] -= s * b.y;
-    v[c + 6] -= s * b.z;
-    v[c + 7] -= s * b.w;
-    __builtin_amdgcn_sched_barrier(0);
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    if (row >= kj && row < n) {
+      const float a = fabsf(pv[q][J]);
+      const uint32_t kk = (a == a) ? __float_as_uint(a) + 1u : (row == kj ? 0xFFFFFFFFu : 0u);
+      if (kk > key) { key = kk; crow = row; }  // rows ascend with q: ties keep the first
+    }
+  }
+  const uint32_t K = wave_umax(key);
+  const int pr = (int)wave_umin(key == K ? (uint32_t)crow : 0xFFFFFFFFu);
+  if (pr != kj) {
+    // row kj sits in lane J (q = 0), row pr in lane lp, slot qp
+    const int lp = (pr - k) & 63, qp = (pr - k) >> 6;
+    if (lane == J) {
+#pragma unroll
+      for (int c = 0; c < kNB; c++) S.rowbuf[0][c] = pv[0][c];
+    }
+    // one block per slot, each marked distinct: merged into one block the
+    // accesses would become pv[qp][c], a dynamic index
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if (q == qp && lane == lp) {
+        asm volatile("; slot %0" ::"n"(q));
+#pragma unroll
+        for (int c = 0; c < kNB; c++) S.rowbuf[1][c] = pv[q][c];
+      }
+    wave_sync();
+    if (lane == J) {
+#pragma unroll
+      for (int c = 0; c < kNB; c++) pv[0][c] = S.rowbuf[1][c];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if (q == qp && lane == lp) {
+        asm volatile("; slot %0" ::"n"(q));
+#pragma unroll
+        for (int c = 0; c < kNB; c++) pv[q][c] = S.rowbuf[0][c];
+      }
+    if (lane == 0) {
+      const int sw = S.perm[kj];
+      S.perm[kj] = S.perm[pr];
+      S.perm[pr] = sw;
+    }
+    wave_sync();
+  }
+  if (lane == 0) S.ipiv[kj] = pr;
+  // the pivot row, now in lane J, read into scalar registers
+  const float praw = readlane_f(pv[0][J], J);
+  if (praw == 0.f && info == 0) info = kj + 1;
+  const float pivot = guard_f(praw);
+  float u[kNB];
+#pragma unroll
+  for (int c = J + 1; c < kNB; c++) u[c] = readlane_f(pv[0][c], J);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    if (row > kj && row < n) {
+      const float l = pv[q][J] / pivot;
+      pv[q][J] = l;
+#pragma unroll
+      for (int c = J + 1; c < kNB; c++) pv[q][c] -= l * u[c];
+    }
   }
 }
 
-__device__ __forceinline__ void store_lds(const float (&v)[64], float* w) {
-#pragma unroll
-  for (int c = 0; c < 64; c += 4) *(float4*)(w + c) = make_float4(v[c], v[c + 1], v[c + 2], v[c + 3]);
-}
-
-__device__ __forceinline__ void load_lds(float (&v)[64], const float* w) {
-#pragma unroll
-  for (int c = 0; c < 64; c += 4) {
-    const float4 a = *(const float4*)(w + c);
-    v[c] = a.x;
-    v[c + 1] = a.y;
-    v[c + 2] = a.z;
-    v[c + 3] = a.w;
-    __builtin_amdgcn_sched_barrier(0);
+template <int Q, int J>
+__device__ __forceinline__ void lu_columns(float (&pv)[Q][kNB], int n, int k, int nb, WaveLds& S, int lane,
+                                           int& info) {
+  if constexpr (J < kNB) {
+    if (J < nb) {
+      lu_column<Q, J>(pv, n, k, S, lane, info);
+      lu_columns<Q, J + 1>(pv, n, k, nb, S, lane, info);
+    }
   }
 }
 
-template <int G>
-__device__ __forceinline__ float group_sum_f(float x) {
+template <int Q>
+__device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+                                         int& info) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float pv[Q][kNB];
 #pragma unroll
-  for (int m = 1; m < G; m <<= 1) x += __shfl_xor(x, m);
-  return x;
-}
-template <int G>
-__device__ __forceinline__ double group_sum_d(double x) {
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    const uint32_t ro = (uint32_t)(row < n ? row : 0) * (uint32_t)n + (uint32_t)k;
 #pragma unroll
-  for (int m = 1; m < G; m <<= 1) x += __shfl_xor(x, m);
-  return x;
-}
-
-__device__ __forceinline__ bool better(float ma, int ra, float mb, int rb) {
-  return ma > mb || (ma == mb && ra < rb);
-}
-
-// (m ? a : b) for a wave-uniform 64-bit mask, as an opaque v_cndmask: a
-// plain ?: over two elements of v is folded into one load from a selected
-// address, which would pin v in scratch memory.
-__device__ __forceinline__ float sel_u(uint64_t m, float a, float b) {
-  float r;
-  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
-  return r;
-}
-
-// v[k] for a uniform k in [0, 64): a six-level select tree (63 selects, no
-// register-indexed access, nothing loop-carried).
-__device__ __forceinline__ float pick64(const float (&v)[64], int k) {
-  float a[32], b[16], c[8], d[4], e[2];
-  const uint64_t m0 = (k & 1) ? ~0ull : 0ull, m1 = (k & 2) ? ~0ull : 0ull, m2 = (k & 4) ? ~0ull : 0ull;
-  const uint64_t m3 = (k & 8) ? ~0ull : 0ull, m4 = (k & 16) ? ~0ull : 0ull, m5 = (k & 32) ? ~0ull : 0ull;
+    for (int c = 0; c < kNB; c++) pv[q][c] = buf_load<float>(rf, ((row < n) & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+  lu_columns<Q, 0>(pv, n, k, nb, S, lane, info);
 #pragma unroll
-  for (int i = 0; i < 32; i++) a[i] = sel_u(m0, v[2 * i + 1], v[2 * i]);
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    if (row < n) {
 #pragma unroll
-  for (int i = 0; i < 16; i++) b[i] = sel_u(m1, a[2 * i + 1], a[2 * i]);
-#pragma unroll
-  for (int i = 0; i < 8; i++) c[i] = sel_u(m2, b[2 * i + 1], b[2 * i]);
-#pragma unroll
-  for (int i = 0; i < 4; i++) d[i] = sel_u(m3, c[2 * i + 1], c[2 * i]);
-#pragma unroll
-  for (int i = 0; i < 2; i++) e[i] = sel_u(m4, d[2 * i + 1], d[2 * i]);
-  return sel_u(m5, e[1], e[0]);
-}
-
-// a[row][k] of the lane's row, for every lane of the row's group
-template <int G>
-__device__ __forceinline__ float column_value(const float (&v)[64], int k, int lane) {
-  const float own = pick64(v, k & 63);
-  if (G == 1) return own;
-  return __shfl(own, (lane & ~(G - 1)) | (k >> 6));
-}
-
-// Pivoted LU of the team's problem (PivotedLU's arithmetic,
-// _array/lu.mojo:165-216: first maximum of |a_ik| over i >= k with the
-// diagonal as the incumbent, full-row interchange, pivot guard_nonzero(a_kk),
-// multiplier a_ik / pivot).  The trailing update only touches columns > k,
-// so a_ik stays in v and the multiplier is formed again, by the same
-// division, when the factor is written out (S.dpiv holds the pivots).
-// S.perm[i] ends as the original row now at row i; S.info the first k with
-// an exactly-zero pivot.
-template <int G>
-__device__ __forceinline__ void lu_team(float (&v)[64], int n, BatchShared<G>& S, int t) {
-  constexpr int NW = Team<G>::NW;
-  const int lane = t & 63, wave = t >> 6, row = t / G, g = t % G;
-  for (int k = 0; k < n; k++) {
-    const float cur = column_value<G>(v, k, lane);
-    float mag = -1.f;
-    int crow = INT_MAX;
-    if (g == 0 && row >= k && row < n) {
-      const float a = fabsf(cur);
-      // NaN never displaces the incumbent; a NaN incumbent is never displaced
-      mag = (a == a) ? a : (row == k ? INFINITY : -1.f);
-      crow = row;
+      for (int c = 0; c < kNB; c++)
+        if (c < nb) F[(int64_t)row * n + k + c] = pv[q][c];
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float om = __shfl_xor(mag, off);
-      const int orr = __shfl_xor(crow, off);
-      if (better(om, orr, mag, crow)) { mag = om; crow = orr; }
-    }
-    if (NW > 1) {
-      if (lane == 0) { S.cmag[wave] = mag; S.crow[wave] = crow; }
-      __syncthreads();
-      mag = lane < NW ? S.cmag[lane] : -1.f;
-      crow = lane < NW ? S.crow[lane] : INT_MAX;
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) {
-        const float om = __shfl_xor(mag, off);
-        const int orr = __shfl_xor(crow, off);
-        if (better(om, orr, mag, crow)) { mag = om; crow = orr; }
-      }
-    }
-    const int pr = __builtin_amdgcn_readfirstlane(crow);
-    if (row == pr) {
-      store_lds(v, S.P + 64 * g);
-      const int lim = k - 64 * g;  // positions <= lim are columns <= k
-#pragma unroll
-      for (int c = 0; c < 64; c += 4) {
-        float4 u;
-        u.x = (c + 0 > lim) ? v[c + 0] : 0.f;
-        u.y = (c + 1 > lim) ? v[c + 1] : 0.f;
-        u.z = (c + 2 > lim) ? v[c + 2] : 0.f;
-        u.w = (c + 3 > lim) ? v[c + 3] : 0.f;
-        *(float4*)(S.U + 64 * g + c) = u;
-      }
-      if (g == 0) S.pcur = cur;
-    }
-    if (pr != k && row == k) {
-      store_lds(v, S.K + 64 * g);
-      if (g == 0) S.kcur = cur;
-    }
-    __syncthreads();
-    if (pr != k) {
-      if (row == k) load_lds(v, S.P + 64 * g);
-      else if (row == pr) load_lds(v, S.K + 64 * g);
-    }
-    const float praw = S.pcur;
-    const float pivot = guard_f(praw);
-    if (t == 0) {
-      S.dpiv[k] = pivot;
-      if (pr != k) {
-        const int sw = S.perm[k];
-        S.perm[k] = S.perm[pr];
-        S.perm[pr] = sw;
-      }
-      if (praw == 0.f && S.info == 0) S.info = k + 1;
-    }
-    // the multiplier of each row below k from its column value after the
-    // interchange: row pr now holds the old row k
-    const bool below = row > k && row < n;
-    const float a_ik = (row == pr && pr != k) ? S.kcur : cur;
-    if (below) axpy_lds(v, a_ik / pivot, S.U + 64 * g);
   }
 }
 
-// Cholesky of the team's problem (lower triangle read): right-looking, the
-// diagonal guard_nonzero'ed before the square root so a negative pivot gives
-// NaN as _array/cholesky.mojo:26-73 does.  Column k's raw values stay in v
-// (updates only touch columns > k); L = raw / ljj is formed at write-out
-// (S.dpiv holds ljj).  S.info gets the first non-positive pivot (1-based).
-template <int G>
-__device__ __forceinline__ void chol_team(float (&v)[64], int n, BatchShared<G>& S, int t) {
-  constexpr int NMAX = Team<G>::NMAX;
-  const int lane = t & 63, row = t / G, g = t % G;
-  for (int k = 0; k < n; k++) {
-    float* col = S.col[k & 1];
-    const float cur = column_value<G>(v, k, lane);
-    if (g == 0 && row < NMAX) {
-      col[row] = (row > k && row < n) ? cur : 0.f;
-      if (row == k) S.pcur = cur;
+// The panel's interchanges ipiv[k..k+nb) over the columns outside it: the
+// net row moves (<= 2 nb rows) are worked out once by the wavefront, then
+// each lane moves whole columns (all loads before the stores).
+template <int Q>
+__device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane) {
+  int myrow = lane < nb ? k + lane : -1;
+  int mine = myrow;
+  int nr = nb;
+  for (int j = 0; j < nb; j++) {
+    const int qrow = S.ipiv[k + j];
+    const unsigned long long hit = __ballot(lane < nr && myrow == qrow);
+    int pos;
+    if (hit) {
+      pos = __ffsll((long long)hit) - 1;
+    } else {
+      pos = nr;
+      if (lane == nr) { myrow = qrow; mine = qrow; }
+      nr++;
     }
-    __syncthreads();
-    const float d = S.pcur;
+    const int cj = __shfl(mine, j);
+    const int cp = __shfl(mine, pos);
+    if (lane == j) mine = cp;
+    if (lane == pos) mine = cj;
+  }
+  const bool moved = lane < nr && mine != myrow;
+  const unsigned long long mm = __ballot(moved);
+  if (moved) {
+    const int at = __popcll(mm & ((1ull << lane) - 1ull));
+    S.mv_dst[at] = myrow;
+    S.mv_src[at] = mine;
+  }
+  const int moves = __popcll(mm);
+  wave_sync();
+  if (moves == 0) return;
+#pragma unroll 1
+  for (int s = 0; s < Q; s++) {
+    const int c = lane + 64 * s;
+    if (c < n && (c < k || c >= k + nb)) {
+      float vals[2 * kNB];
+#pragma unroll
+      for (int e = 0; e < 2 * kNB; e++) vals[e] = F[(int64_t)S.mv_src[e < moves ? e : moves - 1] * n + c];
+#pragma unroll
+      for (int e = 0; e < 2 * kNB; e++)
+        if (e < moves) F[(int64_t)S.mv_dst[e] * n + c] = vals[e];
+    }
+  }
+  wave_sync();
+}
+
+// U12 := L11^-1 A12 over columns k+nb..n.  Lane i < nb holds row i of the
+// unit-lower L11, so each entry reaches every lane as a scalar read (an LDS
+// copy read by all lanes gets all 496 reads hoisted ahead of the arithmetic
+// and spills), and is applied to the lane's Q columns.
+template <int Q>
+__device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, int nb, int lane) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float l11[kNB];
+  {
+    const uint32_t ro = (uint32_t)(k + (lane < nb ? lane : 0)) * (uint32_t)n + (uint32_t)k;
+#pragma unroll
+    for (int c = 0; c < kNB; c++) l11[c] = buf_load<float>(rf, ((lane < nb) & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+  float x[Q][kNB];
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int c = k + nb + lane + 64 * s;
+#pragma unroll
+    for (int i = 0; i < kNB; i++)
+      x[s][i] = buf_load<float>(rf, ((c < n) & (i < nb)) ? 4u * ((uint32_t)(k + i) * (uint32_t)n + (uint32_t)c) : kBufOob);
+  }
+#pragma unroll
+  for (int i = 1; i < kNB; i++) {
+#pragma unroll
+    for (int jj = 0; jj < i; jj++) {
+      const float l = readlane_f(l11[jj], i);
+#pragma unroll
+      for (int s = 0; s < Q; s++) x[s][i] -= l * x[s][jj];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int c = k + nb + lane + 64 * s;
+    if (c < n) {
+#pragma unroll
+      for (int i = 0; i < kNB; i++)
+        if (i < nb) F[(int64_t)(k + i) * n + c] = x[s][i];
+    }
+  }
+  wave_sync();
+}
+
+// ---- Cholesky panel: the diagonal block in lanes 0..nb, then L21 ------------
+// _array/cholesky.mojo:26-73's arithmetic per column: the diagonal
+// guard_nonzero'ed before the square root (a negative pivot gives NaN), the
+// column divided by ljj; right-looking here.
+template <int Q>
+__device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+                                           int& info) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float dv[kNB];
+  {
+    const int row = k + lane;
+    const bool own = lane < nb;
+    const uint32_t ro = (uint32_t)(own ? row : 0) * (uint32_t)n + (uint32_t)k;
+#pragma unroll
+    for (int c = 0; c < kNB; c++) dv[c] = buf_load<float>(rf, (own & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+#pragma unroll
+  for (int j = 0; j < kNB; j++) {
+    if (j >= nb) break;
+    const float d = readlane_f(dv[j], j);
+    if (!(d > 0.f) && info == 0) info = k + j + 1;
     const float ljj = sqrtf(guard_f(d));
-    if (t == 0) {
-      S.dpiv[k] = ljj;
-      if (!(d > 0.f) && S.info == 0) S.info = k + 1;
-    }
-    if (row > k && row < n) {
-      const float sc = (cur / ljj) / ljj;
-      const float* cv = col + 64 * g;
+    if (lane == j) dv[j] = ljj;
+    else if (lane > j) dv[j] = dv[j] / ljj;
 #pragma unroll
-      for (int c = 0; c < 64; c += 8) {
-#pragma unroll
-        for (int i = c; i < c + 8; i++) v[i] -= sc * cv[i];
-        __builtin_amdgcn_sched_barrier(0);
-      }
+    for (int c = j + 1; c < kNB; c++) {
+      const float lc = readlane_f(dv[j], c);
+      if (lane > j) dv[c] -= dv[j] * lc;
     }
   }
+  if (lane < nb) {
+#pragma unroll
+    for (int c = 0; c < kNB; c++)
+      if (c < nb) F[(int64_t)(k + lane) * n + k + c] = c <= lane ? dv[c] : 0.f;
+  }
+  // L21 := A21 L11^-T, one row per lane and slot; L11[c][jj] reaches every
+  // lane as a scalar read of lane c's register jj
+  float x[Q][kNB];
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int row = k + nb + lane + 64 * s;
+#pragma unroll
+    for (int c = 0; c < kNB; c++)
+      x[s][c] = buf_load<float>(rf, ((row < n) & (c < nb)) ? 4u * ((uint32_t)row * (uint32_t)n + (uint32_t)(k + c))
+                                                          : kBufOob);
+  }
+#pragma unroll
+  for (int c = 0; c < kNB; c++) {
+#pragma unroll
+    for (int jj = 0; jj < c; jj++) {
+      const float l = readlane_f(dv[jj], c);
+#pragma unroll
+      for (int s = 0; s < Q; s++) x[s][c] -= x[s][jj] * l;
+    }
+    const float d = readlane_f(dv[c], c);
+#pragma unroll
+    for (int s = 0; s < Q; s++) x[s][c] = x[s][c] / d;
+  }
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int row = k + nb + lane + 64 * s;
+    if (row < n) {
+#pragma unroll
+      for (int c = 0; c < kNB; c++)
+        if (c < nb) F[(int64_t)row * n + k + c] = x[s][c];
+    }
+  }
+  wave_sync();
 }
 
-// op(F) x = xs in place (LDS), F the team's FP32 factor (row-major n x n).
-// LOWER walks top-down, else bottom-up; TRANS reads F transposed (L^T).
-// Diagonal blocks of 64 are solved by wavefront 0 with every index static;
-// the other rows are updated blockwise by the whole team.
-template <int G, bool LOWER, bool UNIT, bool TRANS>
-__device__ __forceinline__ void tri_solve_team(const float* __restrict__ F, int n, BatchShared<G>& S, int t) {
-  const int lane = t & 63, wave = t >> 6, row = t / G, g = t % G;
-  const int nb = (n + 63) / 64;
-  for (int bb = 0; bb < nb; bb++) {
-    const int b = LOWER ? bb : nb - 1 - bb;
+// ---- trailing update on MFMA -----------------------------------------------
+// C -= L21 U12 (LU) or the lower tiles of C -= L21 L21^T (Cholesky), 32 x 32
+// tiles of v_mfma_f32_32x32x2f32 (A element: lane l holds A[l%32][l/32]; B:
+// B[l/32][l%32]; D item i: row 8 (i/4) + 4 (l/32) + i%4, column l%32).
+template <bool CHOL>
+__device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, int k, int nb, int lane) {
+  const int r0 = k + nb;
+  const int m2 = n - r0;
+  if (m2 <= 0) return;
+  const int tiles = (m2 + 31) >> 5;
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  const int l32 = lane & 31, kh = lane >> 5;
+#pragma unroll 1
+  for (int tj = 0; tj < tiles; tj++) {
+    const int col = r0 + 32 * tj + l32;
+    float b[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+      const int kk = 2 * s + kh;
+      const uint32_t off = CHOL ? (uint32_t)col * (uint32_t)n + (uint32_t)(k + kk)
+                                : (uint32_t)(k + kk) * (uint32_t)n + (uint32_t)col;
+      b[s] = buf_load<float>(rf, ((kk < nb) & (col < n)) ? 4u * off : kBufOob);
+    }
+    // tile ti's operands are loaded while tile ti-1 is on the matrix cores
+    auto load_a = [&](int ti, float (&a)[16]) {
+      const int arow = r0 + 32 * ti + l32;
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const int kk = 2 * s + kh;
+        a[s] = buf_load<float>(rf, ((kk < nb) & (arow < n) & (ti < tiles))
+                                       ? 4u * ((uint32_t)arow * (uint32_t)n + (uint32_t)(k + kk))
+                                       : kBufOob);
+      }
+    };
+    auto load_c = [&](int ti, floatx16& c) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = r0 + 32 * ti + 8 * (i >> 2) + 4 * kh + (i & 3);
+        c[i] = buf_load<float>(rf, ((row < n) & (col < n) & (ti < tiles))
+                                       ? 4u * ((uint32_t)row * (uint32_t)n + (uint32_t)col)
+                                       : kBufOob);
+      }
+    };
+    const int t0 = CHOL ? tj : 0;
+    float a[16], an[16];
+    floatx16 acc, cn;
+    load_a(t0, a);
+    load_c(t0, acc);
+#pragma unroll 1
+    for (int ti = t0; ti < tiles; ti++) {
+      load_a(ti + 1, an);
+      load_c(ti + 1, cn);
+#pragma unroll
+      for (int s = 0; s < 16; s++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(-a[s], b[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = r0 + 32 * ti + 8 * (i >> 2) + 4 * kh + (i & 3);
+        if (row < n && col < n) F[(int64_t)row * n + col] = acc[i];
+      }
+#pragma unroll
+      for (int s = 0; s < 16; s++) a[s] = an[s];
+      acc = cn;
+    }
+  }
+  wave_sync();
+}
+
+// op(F) x = xs in place, 64 rows at a time: the diagonal block with every
+// index static (values exchanged through scalar reads), then the other rows
+// by one dot product per lane and slot.  TRANS reads F transposed (L^T).
+template <int Q, bool LOWER, bool UNIT, bool TRANS>
+__device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, WaveLds& S, int lane) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  const int nbk = (n + 63) >> 6;
+#pragma unroll 1
+  for (int bb = 0; bb < nbk; bb++) {
+    const int b = LOWER ? bb : nbk - 1 - bb;
     const int r0 = 64 * b, bs = min(64, n - r0);
-    if (wave == 0) {
+    {
       float a[64];
-      const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F + (int64_t)r0 * n + r0);
 #pragma unroll
       for (int c = 0; c < 64; c++) {
-        // op(F)[r0 + lane][r0 + c]
-        const uint32_t off = TRANS ? (uint32_t)(c * n + lane) : (uint32_t)(lane * n + c);
+        const uint32_t off = TRANS ? (uint32_t)(r0 + c) * (uint32_t)n + (uint32_t)(r0 + lane)
+                                   : (uint32_t)(r0 + lane) * (uint32_t)n + (uint32_t)(r0 + c);
         a[c] = buf_load<float>(rf, ((lane < bs) & (c < bs)) ? 4u * off : kBufOob);
       }
       float acc = lane < bs ? S.xs[r0 + lane] : 0.f;
-      // an opaque copy of the lane index: otherwise the 128 lane comparisons
-      // below are hoisted out of the block loop and spilled
+      // an opaque lane index keeps the 128 comparisons below from being hoisted
       int ln = lane;
       asm volatile("" : "+v"(ln));
       if (LOWER) {
@@ -357,176 +483,209 @@ __device__ __forceinline__ void tri_solve_team(const float* __restrict__ F, int 
       }
       if (lane < bs) S.xs[r0 + lane] = acc;
     }
-    __syncthreads();
-    const bool todo = row < n && (LOWER ? row >= r0 + bs : row < r0);
-    float part = 0.f;
-    if (todo) {
-      constexpr int CW = 64 / G;  // block columns per lane
-      const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+    wave_sync();
+    {
+      float part[Q];
+      bool todo[Q];
 #pragma unroll
-      for (int i0 = 0; i0 < CW; i0 += 8) {
-#pragma unroll
-        for (int i = i0; i < i0 + 8; i++) {
-          const int c = g * CW + i;
-          const uint32_t off = TRANS ? (uint32_t)((r0 + c) * n + row) : (uint32_t)(row * n + r0 + c);
-          part += buf_load<float>(rf, c < bs ? 4u * off : kBufOob) * S.xs[r0 + (c < bs ? c : 0)];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+      for (int s = 0; s < Q; s++) {
+        const int row = lane + 64 * s;
+        todo[s] = row < n && (LOWER ? row >= r0 + bs : row < r0);
+        part[s] = 0.f;
       }
+#pragma unroll 1
+      for (int c0 = 0; c0 < bs; c0 += 16) {
+        float fv[Q][16];
+#pragma unroll
+        for (int s = 0; s < Q; s++) {
+          const int row = lane + 64 * s;
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int c = c0 + u;
+            const uint32_t off = TRANS ? (uint32_t)(r0 + c) * (uint32_t)n + (uint32_t)row
+                                       : (uint32_t)row * (uint32_t)n + (uint32_t)(r0 + c);
+            fv[s][u] = buf_load<float>(rf, (todo[s] & (c < bs)) ? 4u * off : kBufOob);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const float xc = S.xs[r0 + (c0 + u < bs ? c0 + u : 0)];
+#pragma unroll
+          for (int s = 0; s < Q; s++) part[s] += fv[s][u] * xc;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < Q; s++)
+        if (todo[s]) S.xs[lane + 64 * s] -= part[s];
     }
-    part = group_sum_f<G>(part);
-    if (todo && g == 0) S.xs[row] -= part;
-    __syncthreads();
+    wave_sync();
   }
 }
 
-// x64 += (or =) op^-1 of the current xs (the FP32 correction)
-template <int G, bool CHOL>
-__device__ __forceinline__ void solve_team(const float* __restrict__ F, int n, BatchShared<G>& S, int t) {
+template <int Q, bool CHOL>
+__device__ __forceinline__ void solve_wave(const float* __restrict__ F, int n, WaveLds& S, int lane) {
   if (CHOL) {
-    tri_solve_team<G, true, false, false>(F, n, S, t);
-    tri_solve_team<G, false, false, true>(F, n, S, t);
+    tri_solve<Q, true, false, false>(F, n, S, lane);
+    tri_solve<Q, false, false, true>(F, n, S, lane);
   } else {
-    tri_solve_team<G, true, true, false>(F, n, S, t);
-    tri_solve_team<G, false, false, false>(F, n, S, t);
+    tri_solve<Q, true, true, false>(F, n, S, lane);
+    tri_solve<Q, false, false, false>(F, n, S, lane);
   }
 }
 
-// r64 = b - A x64 (FP64), and the backward-error norms.  A Cholesky problem
-// is the symmetric matrix its lower triangle defines.
-template <int G, bool CHOL>
-__device__ __forceinline__ double residual_team(const double* __restrict__ A, const double* __restrict__ b, int n,
-                                bool first, double& anorm, BatchShared<G>& S, int t) {
-  const int row = t / G, g = t % G;
-  if (t < 4) S.norms[t] = 0ull;
-  __syncthreads();
-  double part = 0.0, apart = 0.0;
-  if (row < n) {
-    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A);
-    const uint32_t rowoff = (uint32_t)row * (uint32_t)n;
-#pragma unroll 16
-    for (int i = 0; i < 64; i++) {
-      const int c = 64 * g + i;
-      double a = buf_load<double>(ra, c < n ? 8u * (rowoff + (uint32_t)c) : kBufOob);
-      if (CHOL && c > row && c < n) a = A[(int64_t)c * n + row];
-      part += a * (c < n ? S.x64[c] : 0.0);
-      apart += fabs(a);
+// r64 = b - A x64 in FP64 (a Cholesky problem is the symmetric matrix its
+// lower triangle defines); returns ||r||_inf / (||A||_inf ||x||_inf).
+template <int Q, bool CHOL>
+__device__ __forceinline__ double residual_wave(const double* __restrict__ A, const double* __restrict__ b, int n,
+                                                bool first, double& anorm, WaveLds& S, int lane) {
+  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A);
+  double rmax = 0.0, xmax = 0.0, amax = 0.0;
+#pragma unroll 1
+  for (int s = 0; s < Q; s++) {
+    const int row = lane + 64 * s;
+    if (row < n) {
+      double part = 0.0, asum = 0.0;
+      const uint32_t ro = (uint32_t)row * (uint32_t)n;
+#pragma unroll 1
+      for (int j0 = 0; j0 < n; j0 += 16) {
+        double av[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int j = j0 + u;
+          const uint32_t off = (CHOL && j > row) ? (uint32_t)j * (uint32_t)n + (uint32_t)row : ro + (uint32_t)j;
+          av[u] = buf_load<double>(ra, j < n ? 8u * off : kBufOob);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int j = j0 + u;
+          part += av[u] * S.x64[j < n ? j : 0];
+          asum += fabs(av[u]);
+        }
+      }
+      const double r = b[row] - part;
+      S.r64[row] = r;
+      rmax = fmax(rmax, fabs(r));
+      xmax = fmax(xmax, fabs(S.x64[row]));
+      amax = fmax(amax, asum);
     }
   }
-  part = group_sum_d<G>(part);
-  apart = group_sum_d<G>(apart);
-  if (row < n && g == 0) {
-    const double r = b[row] - part;
-    S.r64[row] = r;
-    atomicMax(&S.norms[0], (unsigned long long)__double_as_longlong(fabs(r)));
-    atomicMax(&S.norms[1], (unsigned long long)__double_as_longlong(fabs(S.x64[row])));
-    if (first) atomicMax(&S.norms[2], (unsigned long long)__double_as_longlong(apart));
-  }
-  __syncthreads();
-  if (first) anorm = __longlong_as_double((long long)S.norms[2]);
-  const double rmax = __longlong_as_double((long long)S.norms[0]);
-  const double xmax = __longlong_as_double((long long)S.norms[1]);
+  rmax = wave_dmax(rmax);
+  xmax = wave_dmax(xmax);
+  if (first) anorm = wave_dmax(amax);
+  wave_sync();
   const double den = anorm * xmax;
   return den > 0.0 ? rmax / den : (rmax == 0.0 ? 0.0 : INFINITY);
 }
 
-template <int G, bool CHOL>
-__global__ void __launch_bounds__(Team<G>::T) batched_solve_kernel(BatchArgs args, int cls) {
-  __shared__ BatchShared<G> S;
-  const int t = threadIdx.x;
-  const int row = t / G;
+template <int Q, bool CHOL>
+__global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs args, int cls) {
+  __shared__ WaveLds lds[kWaves];
+  const int lane = threadIdx.x & 63;
+  // the wavefront index as a scalar, so every LDS address below is one base
+  // register plus an immediate offset
+  WaveLds& S = lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
   const int begin = args.cls[cls], count = args.cls[kClasses + cls];
   for (;;) {
-    if (t == 0) S.slot = atomicAdd(&args.head[cls], 1);
-    __syncthreads();
-    const int slot = S.slot;
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(&args.head[cls], 1);
+    slot = __shfl(slot, 0);
     if (slot >= count) break;
     const int p = args.order[begin + slot];
     const int n = args.n[p];
     const double* A = args.A + args.offA[p];
     const double* b = args.B + args.offB[p];
     float* F = args.F + args.offA[p];
-    if (t == 0) S.info = 0;
-    if (row < Team<G>::NMAX && (t % G) == 0) S.perm[row] = row;
 
-    // load (FP64 -> FP32 registers); a Cholesky problem reads its lower triangle.
-    // rel = row - 64 g, made opaque per problem: otherwise the 2 x 64
-    // comparisons against it are hoisted out of the problem loop and spilled.
-    int rel = row - 64 * (t % G);
-    asm volatile("" : "+v"(rel));
-    float v[64];
+    // FP32 working copy (a Cholesky problem keeps its lower triangle);
+    // 32 loads in flight per lane, then the stores
     {
-      const int g = t % G;
-      // one descriptor per problem (uniform); the byte offsets stay < 512 KiB
       const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A);
-      const uint32_t rowoff = (uint32_t)(row < n ? row : 0) * (uint32_t)n;
-      // sixteen loads in flight at a time: all 64 FP64 loads at once would
-      // need 128 registers on top of the 64 they land in
+      constexpr int RB = 32 / Q;  // rows per batch
+#pragma unroll 1
+      for (int i0 = 0; i0 < n; i0 += RB) {
+        double v[RB][Q];
 #pragma unroll
-      for (int i0 = 0; i0 < 64; i0 += 16) {
+        for (int r = 0; r < RB; r++) {
+          const int i = i0 + r;
 #pragma unroll
-        for (int i = i0; i < i0 + 16; i++) {
-          const int c = 64 * g + i;
-          const bool ok = (row < n) & (c < n) & (!CHOL | (i <= rel));
-          v[i] = (float)buf_load<double>(ra, ok ? 8u * (rowoff + (uint32_t)c) : kBufOob);
+          for (int s = 0; s < Q; s++) {
+            const int j = lane + 64 * s;
+            const bool keep = (i < n) & (j < n) & (!CHOL | (j <= i));
+            v[r][s] = buf_load<double>(ra, keep ? 8u * ((uint32_t)i * (uint32_t)n + (uint32_t)j) : kBufOob);
+          }
         }
-        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const int i = i0 + r;
+#pragma unroll
+          for (int s = 0; s < Q; s++) {
+            const int j = lane + 64 * s;
+            if ((i < n) & (j < n)) F[(int64_t)i * n + j] = (float)v[r][s];
+          }
+        }
+      }
+      for (int s = 0; s < Q; s++) {
+        const int i = lane + 64 * s;
+        if (i < n) S.perm[i] = i;
       }
     }
-    __syncthreads();
-    if (CHOL)
-      chol_team<G>(v, n, S, t);
-    else
-      lu_team<G>(v, n, S, t);
-    __syncthreads();  // the last column's pivot (S.dpiv) is read below
-    // the FP32 factor: multipliers (raw / pivot) below the diagonal, U (LU)
-    // or ljj on it (Cholesky), zero above it for Cholesky
-    if (row < n) {
-      const int g = t % G;
-#pragma unroll
-      for (int i0 = 0; i0 < 64; i0 += 8) {
-#pragma unroll
-        for (int i = i0; i < i0 + 8; i++) {
-          const int c = 64 * g + i;
-          float f = v[i];
-          if (i < rel) f = f / S.dpiv[c];
-          else if (CHOL) f = (i == rel) ? S.dpiv[c] : 0.f;
-          if (c < n) F[(int64_t)row * n + c] = f;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-    __syncthreads();
+    wave_sync();
 
-    // x0 = op^-1 P b in FP32, then FP64 refinement; one solve site (the
-    // triangular solves are large once unrolled)
+    int info = 0;
+#pragma unroll 1
+    for (int k = 0; k < n; k += kNB) {
+      const int nb = min(kNB, n - k);
+      if (CHOL) {
+        panel_chol<Q>(F, n, k, nb, S, lane, info);
+      } else {
+        panel_lu<Q>(F, n, k, nb, S, lane, info);
+        wave_sync();
+        swap_outside<Q>(F, n, k, nb, S, lane);
+        if (k + nb < n) trsm_u12<Q>(F, n, k, nb, lane);
+      }
+      trailing_update<CHOL>(F, n, k, nb, lane);
+    }
+
+    // x0 = op^-1 P b in FP32, then FP64 refinement (one solve site)
     double anorm = 0.0, eta = 0.0, prev = INFINITY;
     int it = 0;
     for (bool first = true;; first = false) {
-      if (row < n && (t % G) == 0) {
-        const int src = CHOL ? row : S.perm[row];
-        S.xs[row] = first ? (float)b[src] : (float)S.r64[src];
+#pragma unroll 1
+      for (int s = 0; s < Q; s++) {
+        const int row = lane + 64 * s;
+        if (row < n) {
+          const int src = CHOL ? row : S.perm[row];
+          S.xs[row] = first ? (float)b[src] : (float)S.r64[src];
+        }
       }
-      __syncthreads();
-      solve_team<G, CHOL>(F, n, S, t);
-      if (row < n && (t % G) == 0) S.x64[row] = first ? (double)S.xs[row] : S.x64[row] + (double)S.xs[row];
-      __syncthreads();
-      eta = residual_team<G, CHOL>(A, b, n, first, anorm, S, t);
+      wave_sync();
+      solve_wave<Q, CHOL>(F, n, S, lane);
+#pragma unroll 1
+      for (int s = 0; s < Q; s++) {
+        const int row = lane + 64 * s;
+        if (row < n) S.x64[row] = first ? (double)S.xs[row] : S.x64[row] + (double)S.xs[row];
+      }
+      wave_sync();
+      eta = residual_wave<Q, CHOL>(A, b, n, first, anorm, S, lane);
       if (!first) it++;
       if (!(eta > args.tol) || it >= args.max_iter || !(eta < 0.5 * prev)) break;
       prev = eta;
     }
-    if (row < n && (t % G) == 0) args.X[args.offB[p] + row] = S.x64[row];
-    if (t == 0) {
-      args.info[p] = S.info;
+#pragma unroll 1
+    for (int s = 0; s < Q; s++) {
+      const int row = lane + 64 * s;
+      if (row < n) args.X[args.offB[p] + row] = S.x64[row];
+    }
+    if (lane == 0) {
+      args.info[p] = info;
       args.iters[p] = it;
       args.berr[p] = eta;
     }
-    __syncthreads();  // S is reused by the next problem
+    wave_sync();
   }
 }
 
-// ---- size-class queue: counting sort by n, largest first ---------------------
+// ---- size-class queue: counting sort by (kind, n), largest first -------------
 // A problem is taken when 1 <= n <= 256, kind is 0 or 1 and its matrix and
 // vectors lie inside the arrays the caller sized (so no kernel can touch
 // memory outside them); any other is reported with info = -1.
@@ -591,10 +750,10 @@ __global__ void zero_i32_kernel(int32_t* p, int count) {
   if (i < count) p[i] = 0;
 }
 
-template <int G, bool CHOL>
-int launch_class(nmx_ctx* ctx, const BatchArgs& a, int cls, int per_cu) {
-  const int grid = ctx->num_cus * per_cu;
-  batched_solve_kernel<G, CHOL><<<grid, Team<G>::T, 0, ctx->stream>>>(a, cls);
+template <int Q, bool CHOL>
+int launch_class(nmx_ctx* ctx, const BatchArgs& a, int per_cu) {
+  const int cls = (CHOL ? kSizeClasses : 0) + (Q - 1);
+  batched_wave_kernel<Q, CHOL><<<ctx->num_cus * per_cu, 64 * kWaves, 0, ctx->stream>>>(a, cls);
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
@@ -605,9 +764,9 @@ NMX_API void nmx_batch_cfg_default(nmx_batch_cfg* cfg) {
   if (!cfg) return;
   cfg->max_iter = 10;
   cfg->tol = 1e-15;
-  cfg->teams_per_cu[0] = 1;
-  cfg->teams_per_cu[1] = 4;
-  cfg->teams_per_cu[2] = 16;
+  cfg->teams_per_cu[0] = 2;  // workgroups of four independent wavefronts per CU, n > 128
+  cfg->teams_per_cu[1] = 2;  // 64 < n <= 128
+  cfg->teams_per_cu[2] = 4;  // n <= 64
 }
 
 NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
@@ -631,10 +790,10 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   nmx_batch_cfg cfg;
   if (cfg_in) cfg = *cfg_in; else nmx_batch_cfg_default(&cfg);
   if (cfg.max_iter < 0 || !(cfg.tol >= 0.0)) return -15;
-  for (int c = 0; c < kSizeClasses; c++)
+  for (int c = 0; c < 3; c++)
     if (cfg.teams_per_cu[c] <= 0 || cfg.teams_per_cu[c] > 64) return -15;
 
-  // workspace: FP32 factors, then the queue
+  // workspace: FP32 working copies, then the queue
   const size_t f_bytes = ((sizeof(float) * (size_t)a_elems + 255) / 256) * 256;
   const size_t q_ints = 2 * (size_t)kBins + 2 * kClasses + kClasses + (size_t)count;
   void* ws = nullptr;
@@ -660,12 +819,14 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   {
     ProfScope ps(ctx, K_BATCH, 0.0, 8.0 * (double)a_elems);
     // largest problems first; LU then Cholesky within a size class
-    NMX_TRY((launch_class<4, false>(ctx, a, 0, cfg.teams_per_cu[0])));
-    NMX_TRY((launch_class<4, true>(ctx, a, kSizeClasses + 0, cfg.teams_per_cu[0])));
-    NMX_TRY((launch_class<2, false>(ctx, a, 1, cfg.teams_per_cu[1])));
-    NMX_TRY((launch_class<2, true>(ctx, a, kSizeClasses + 1, cfg.teams_per_cu[1])));
-    NMX_TRY((launch_class<1, false>(ctx, a, 2, cfg.teams_per_cu[2])));
-    NMX_TRY((launch_class<1, true>(ctx, a, kSizeClasses + 2, cfg.teams_per_cu[2])));
+    NMX_TRY((launch_class<4, false>(ctx, a, cfg.teams_per_cu[0])));
+    NMX_TRY((launch_class<4, true>(ctx, a, cfg.teams_per_cu[0])));
+    NMX_TRY((launch_class<3, false>(ctx, a, cfg.teams_per_cu[0])));
+    NMX_TRY((launch_class<3, true>(ctx, a, cfg.teams_per_cu[0])));
+    NMX_TRY((launch_class<2, false>(ctx, a, cfg.teams_per_cu[1])));
+    NMX_TRY((launch_class<2, true>(ctx, a, cfg.teams_per_cu[1])));
+    NMX_TRY((launch_class<1, false>(ctx, a, cfg.teams_per_cu[2])));
+    NMX_TRY((launch_class<1, true>(ctx, a, cfg.teams_per_cu[2])));
   }
   return NMX_OK;
 }

@@ -43,10 +43,6 @@ __device__ __forceinline__ float halving_sum(float (&s)[NV], int lane) {
   return v;
 }
 
-__device__ __forceinline__ float readlane_f(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
 
 // One reflector: column K of the stacked [triangle; new rows], applied to
