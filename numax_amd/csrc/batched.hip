@@ -29,6 +29,8 @@
 #include <limits.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "nmx_internal.h"
 
 namespace {
@@ -94,8 +96,313 @@ struct WaveLds {
   float rowbuf[2][kNB];          // interchange staging
   int mv_dst[2 * kNB], mv_src[2 * kNB];
 };
+// the rolled solves (n <= 128) stage each 64 x 64 diagonal block in LDS
+struct WaveLdsBlk : WaveLds {
+  float blk[64][65];
+};
+
+// Every row held in registers is rotated left by one column after each
+// step of a rolled column loop, so the current column is always register 0
+// and all indices stay static: fully unrolled, these loops would be hundreds
+// of KB of straight-line code that each wavefront streams through the
+// instruction cache once per panel.
+template <int N>
+__device__ __forceinline__ void rot1(float (&v)[N]) {
+  const float f = v[0];
+#pragma unroll
+  for (int c = 0; c + 1 < N; c++) v[c] = v[c + 1];
+  v[N - 1] = f;
+}
 
 // ---- panel: pivoted LU of F[k:n, k:k+nb] in registers ------------------------
+// PivotedLU's arithmetic (_array/lu.mojo:165-216): first maximum of |a_ik|
+// with the diagonal as the incumbent (a NaN incumbent is kept, a NaN never
+// displaces one), full-row interchange, pivot guard_nonzero(a_kk),
+// multipliers by true division.
+template <int Q>
+__device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+                                         int& info) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float pv[Q][kNB];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    const uint32_t ro = (uint32_t)(row < n ? row : 0) * (uint32_t)n + (uint32_t)k;
+#pragma unroll
+    for (int c = 0; c < kNB; c++) pv[q][c] = buf_load<float>(rf, ((row < n) & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+#pragma unroll 1
+  for (int j = 0; j < nb; j++) {
+    const int kj = k + j;
+    uint32_t key = 0;
+    int crow = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int row = k + lane + 64 * q;
+      if (row >= kj && row < n) {
+        const float a = fabsf(pv[q][0]);
+        const uint32_t kk = (a == a) ? __float_as_uint(a) + 1u : (row == kj ? 0xFFFFFFFFu : 0u);
+        if (kk > key) { key = kk; crow = row; }  // rows ascend with q: ties keep the first
+      }
+    }
+    const uint32_t K = wave_umax(key);
+    const int pr = (int)wave_umin(key == K ? (uint32_t)crow : 0xFFFFFFFFu);
+    if (pr != kj) {
+      // row kj sits in lane j (slot 0), row pr in lane lp, slot qp
+      const int lp = (pr - k) & 63, qp = (pr - k) >> 6;
+      if (lane == j) {
+#pragma unroll
+        for (int c = 0; c < kNB; c++) S.rowbuf[0][c] = pv[0][c];
+      }
+      // one block per slot, each marked distinct: merged into one block the
+      // accesses would become pv[qp][c], a dynamic index
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if (q == qp && lane == lp) {
+          asm volatile("; slot %0" ::"n"(q));
+#pragma unroll
+          for (int c = 0; c < kNB; c++) S.rowbuf[1][c] = pv[q][c];
+        }
+      wave_sync();
+      if (lane == j) {
+#pragma unroll
+        for (int c = 0; c < kNB; c++) pv[0][c] = S.rowbuf[1][c];
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if (q == qp && lane == lp) {
+          asm volatile("; slot %0" ::"n"(q));
+#pragma unroll
+          for (int c = 0; c < kNB; c++) pv[q][c] = S.rowbuf[0][c];
+        }
+      if (lane == 0) {
+        const int sw = S.perm[kj];
+        S.perm[kj] = S.perm[pr];
+        S.perm[pr] = sw;
+      }
+      wave_sync();
+    }
+    if (lane == 0) S.ipiv[kj] = pr;
+    // the pivot row (lane j) into scalar registers; positions >= nb - j are
+    // finished columns rotated to the end and must not be updated
+    const float praw = readlane_f(pv[0][0], j);
+    if (praw == 0.f && info == 0) info = kj + 1;
+    const float pivot = guard_f(praw);
+    const int rem = nb - j;
+    float u[kNB];
+#pragma unroll
+    for (int c = 1; c < kNB; c++) u[c] = c < rem ? readlane_f(pv[0][c], j) : 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int row = k + lane + 64 * q;
+      if (row > kj && row < n) {
+        const float l = pv[q][0] / pivot;
+        pv[q][0] = l;
+#pragma unroll
+        for (int c = 1; c < kNB; c++) pv[q][c] -= l * u[c];
+      }
+      rot1(pv[q]);
+    }
+  }
+#pragma unroll 1
+  for (int j = nb; j < kNB; j++) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) rot1(pv[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int row = k + lane + 64 * q;
+    if (row < n) {
+#pragma unroll
+      for (int c = 0; c < kNB; c++)
+        if (c < nb) F[(int64_t)row * n + k + c] = pv[q][c];
+    }
+  }
+}
+
+// The panel's interchanges ipiv[k..k+nb) over the columns outside it: the
+// net row moves (<= 2 nb rows) are worked out once by the wavefront, then
+// each lane moves whole columns (all loads before the stores).
+template <int Q>
+__device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane) {
+  int myrow = lane < nb ? k + lane : -1;
+  int mine = myrow;
+  int nr = nb;
+  for (int j = 0; j < nb; j++) {
+    const int qrow = S.ipiv[k + j];
+    const unsigned long long hit = __ballot(lane < nr && myrow == qrow);
+    int pos;
+    if (hit) {
+      pos = __ffsll((long long)hit) - 1;
+    } else {
+      pos = nr;
+      if (lane == nr) { myrow = qrow; mine = qrow; }
+      nr++;
+    }
+    const int cj = __shfl(mine, j);
+    const int cp = __shfl(mine, pos);
+    if (lane == j) mine = cp;
+    if (lane == pos) mine = cj;
+  }
+  const bool moved = lane < nr && mine != myrow;
+  const unsigned long long mm = __ballot(moved);
+  if (moved) {
+    const int at = __popcll(mm & ((1ull << lane) - 1ull));
+    S.mv_dst[at] = myrow;
+    S.mv_src[at] = mine;
+  }
+  const int moves = __popcll(mm);
+  wave_sync();
+  if (moves == 0) return;
+#pragma unroll 1
+  for (int s = 0; s < Q; s++) {
+    const int c = lane + 64 * s;
+    if (c < n && (c < k || c >= k + nb)) {
+      float vals[2 * kNB];
+#pragma unroll
+      for (int e = 0; e < 2 * kNB; e++) vals[e] = F[(int64_t)S.mv_src[e < moves ? e : moves - 1] * n + c];
+#pragma unroll
+      for (int e = 0; e < 2 * kNB; e++)
+        if (e < moves) F[(int64_t)S.mv_dst[e] * n + c] = vals[e];
+    }
+  }
+  wave_sync();
+}
+
+// U12 := L11^-1 A12 over columns k+nb..n, right-looking over the rows of
+// L11 with the lane's columns (x) and L11's rows (lane i < nb holds row i)
+// both rotated, so L[jj + c][jj] is lane jj + c's register 0.
+template <int Q>
+__device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, int nb, int lane) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float l11[kNB];
+  {
+    const uint32_t ro = (uint32_t)(k + (lane < nb ? lane : 0)) * (uint32_t)n + (uint32_t)k;
+#pragma unroll
+    for (int c = 0; c < kNB; c++) l11[c] = buf_load<float>(rf, ((lane < nb) & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+  float x[Q][kNB];
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int c = k + nb + lane + 64 * s;
+#pragma unroll
+    for (int i = 0; i < kNB; i++)
+      x[s][i] = buf_load<float>(rf, ((c < n) & (i < nb)) ? 4u * ((uint32_t)(k + i) * (uint32_t)n + (uint32_t)c) : kBufOob);
+  }
+#pragma unroll 1
+  for (int jj = 0; jj < nb; jj++) {
+    const int rem = nb - jj;
+#pragma unroll
+    for (int c = 1; c < kNB; c++) {
+      const float l = c < rem ? readlane_f(l11[0], jj + c) : 0.f;
+#pragma unroll
+      for (int s = 0; s < Q; s++) x[s][c] -= l * x[s][0];
+    }
+#pragma unroll
+    for (int s = 0; s < Q; s++) rot1(x[s]);
+    rot1(l11);
+  }
+#pragma unroll 1
+  for (int jj = nb; jj < kNB; jj++) {
+#pragma unroll
+    for (int s = 0; s < Q; s++) rot1(x[s]);
+  }
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int c = k + nb + lane + 64 * s;
+    if (c < n) {
+#pragma unroll
+      for (int i = 0; i < kNB; i++)
+        if (i < nb) F[(int64_t)(k + i) * n + c] = x[s][i];
+    }
+  }
+  wave_sync();
+}
+
+// ---- Cholesky panel: the diagonal block in lanes 0..nb, then L21 ------------
+// _array/cholesky.mojo:26-73's arithmetic per column: the diagonal
+// guard_nonzero'ed before the square root (a negative pivot gives NaN), the
+// column divided by ljj; right-looking here, rows rotated as in the LU panel.
+template <int Q>
+__device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+                                           int& info) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  float dv[kNB];
+  {
+    const bool own = lane < nb;
+    const uint32_t ro = (uint32_t)(k + (own ? lane : 0)) * (uint32_t)n + (uint32_t)k;
+#pragma unroll
+    for (int c = 0; c < kNB; c++) dv[c] = buf_load<float>(rf, (own & (c < nb)) ? 4u * (ro + c) : kBufOob);
+  }
+#pragma unroll 1
+  for (int j = 0; j < nb; j++) {
+    const float d = readlane_f(dv[0], j);
+    if (!(d > 0.f) && info == 0) info = k + j + 1;
+    const float ljj = sqrtf(guard_f(d));
+    if (lane == j) dv[0] = ljj;
+    else if (lane > j) dv[0] = dv[0] / ljj;
+    const int rem = nb - j;
+#pragma unroll
+    for (int c = 1; c < kNB; c++) {
+      const float lc = c < rem ? readlane_f(dv[0], j + c) : 0.f;
+      if (lane > j) dv[c] -= dv[0] * lc;
+    }
+    rot1(dv);
+  }
+#pragma unroll 1
+  for (int j = nb; j < kNB; j++) rot1(dv);
+  if (lane < nb) {
+#pragma unroll
+    for (int c = 0; c < kNB; c++)
+      if (c < nb) F[(int64_t)(k + lane) * n + k + c] = c <= lane ? dv[c] : 0.f;
+  }
+  // L21 := A21 L11^-T, right-looking over the columns of L11 with x and
+  // L11's rows rotated: L[jj + c][jj] is lane jj + c's register 0
+  float x[Q][kNB];
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int row = k + nb + lane + 64 * s;
+#pragma unroll
+    for (int c = 0; c < kNB; c++)
+      x[s][c] = buf_load<float>(rf, ((row < n) & (c < nb)) ? 4u * ((uint32_t)row * (uint32_t)n + (uint32_t)(k + c))
+                                                          : kBufOob);
+  }
+#pragma unroll 1
+  for (int jj = 0; jj < nb; jj++) {
+    const float djj = readlane_f(dv[0], jj);
+#pragma unroll
+    for (int s = 0; s < Q; s++) x[s][0] = x[s][0] / djj;
+    const int rem = nb - jj;
+#pragma unroll
+    for (int c = 1; c < kNB; c++) {
+      const float l = c < rem ? readlane_f(dv[0], jj + c) : 0.f;
+#pragma unroll
+      for (int s = 0; s < Q; s++) x[s][c] -= x[s][0] * l;
+    }
+#pragma unroll
+    for (int s = 0; s < Q; s++) rot1(x[s]);
+    rot1(dv);
+  }
+#pragma unroll 1
+  for (int jj = nb; jj < kNB; jj++) {
+#pragma unroll
+    for (int s = 0; s < Q; s++) rot1(x[s]);
+  }
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int row = k + nb + lane + 64 * s;
+    if (row < n) {
+#pragma unroll
+      for (int c = 0; c < kNB; c++)
+        if (c < nb) F[(int64_t)row * n + k + c] = x[s][c];
+    }
+  }
+  wave_sync();
+}
+
+// ---- fully unrolled panel, U12 and solve variants (n > 128) -------------------
+// Every column index static, the pivot row and L11 exchanged through scalar
+// reads; the classes with n > 128 run these.
 // PivotedLU's arithmetic (_array/lu.mojo:165-216): first maximum of |a_ik|
 // with the diagonal as the incumbent (a NaN incumbent is kept, a NaN never
 // displaces one), full-row interchange, pivot guard_nonzero(a_kk),
@@ -186,7 +493,7 @@ __device__ __forceinline__ void lu_columns(float (&pv)[Q][kNB], int n, int k, in
 }
 
 template <int Q>
-__device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+__device__ __forceinline__ void panel_lu_unrolled(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
                                          int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float pv[Q][kNB];
@@ -209,61 +516,12 @@ __device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, in
   }
 }
 
-// The panel's interchanges ipiv[k..k+nb) over the columns outside it: the
-// net row moves (<= 2 nb rows) are worked out once by the wavefront, then
-// each lane moves whole columns (all loads before the stores).
-template <int Q>
-__device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane) {
-  int myrow = lane < nb ? k + lane : -1;
-  int mine = myrow;
-  int nr = nb;
-  for (int j = 0; j < nb; j++) {
-    const int qrow = S.ipiv[k + j];
-    const unsigned long long hit = __ballot(lane < nr && myrow == qrow);
-    int pos;
-    if (hit) {
-      pos = __ffsll((long long)hit) - 1;
-    } else {
-      pos = nr;
-      if (lane == nr) { myrow = qrow; mine = qrow; }
-      nr++;
-    }
-    const int cj = __shfl(mine, j);
-    const int cp = __shfl(mine, pos);
-    if (lane == j) mine = cp;
-    if (lane == pos) mine = cj;
-  }
-  const bool moved = lane < nr && mine != myrow;
-  const unsigned long long mm = __ballot(moved);
-  if (moved) {
-    const int at = __popcll(mm & ((1ull << lane) - 1ull));
-    S.mv_dst[at] = myrow;
-    S.mv_src[at] = mine;
-  }
-  const int moves = __popcll(mm);
-  wave_sync();
-  if (moves == 0) return;
-#pragma unroll 1
-  for (int s = 0; s < Q; s++) {
-    const int c = lane + 64 * s;
-    if (c < n && (c < k || c >= k + nb)) {
-      float vals[2 * kNB];
-#pragma unroll
-      for (int e = 0; e < 2 * kNB; e++) vals[e] = F[(int64_t)S.mv_src[e < moves ? e : moves - 1] * n + c];
-#pragma unroll
-      for (int e = 0; e < 2 * kNB; e++)
-        if (e < moves) F[(int64_t)S.mv_dst[e] * n + c] = vals[e];
-    }
-  }
-  wave_sync();
-}
-
 // U12 := L11^-1 A12 over columns k+nb..n.  Lane i < nb holds row i of the
 // unit-lower L11, so each entry reaches every lane as a scalar read (an LDS
 // copy read by all lanes gets all 496 reads hoisted ahead of the arithmetic
 // and spills), and is applied to the lane's Q columns.
 template <int Q>
-__device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, int nb, int lane) {
+__device__ __forceinline__ void trsm_u12_unrolled(float* __restrict__ F, int n, int k, int nb, int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float l11[kNB];
   {
@@ -305,7 +563,7 @@ __device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, in
 // guard_nonzero'ed before the square root (a negative pivot gives NaN), the
 // column divided by ljj; right-looking here.
 template <int Q>
-__device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+__device__ __forceinline__ void panel_chol_unrolled(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
                                            int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float dv[kNB];
@@ -437,11 +695,90 @@ __device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, in
   wave_sync();
 }
 
+// op(F) x = xs in place, 64 rows at a time: the diagonal block is staged
+// in LDS and swept by a rolled loop (lane c's value reaches the others as a
+// scalar read), then the other rows are updated by one dot product per lane
+// and slot.  TRANS reads F transposed (L^T).
+template <int Q, bool LOWER, bool UNIT, bool TRANS>
+__device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, WaveLdsBlk& S, int lane) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  const int nbk = (n + 63) >> 6;
+#pragma unroll 1
+  for (int bb = 0; bb < nbk; bb++) {
+    const int b = LOWER ? bb : nbk - 1 - bb;
+    const int r0 = 64 * b, bs = min(64, n - r0);
+    // stage op(F)[r0 + i][r0 + c] at blk[i][c]; F's rows are read along the
+    // lanes (coalesced) and TRANS stores them as columns of blk
+#pragma unroll 1
+    for (int i0 = 0; i0 < 64; i0 += 16) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int i = i0 + u;
+        const uint32_t off = (uint32_t)(r0 + i) * (uint32_t)n + (uint32_t)(r0 + lane);
+        v[u] = buf_load<float>(rf, ((lane < bs) & (i < bs)) ? 4u * off : kBufOob);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        if (TRANS) S.blk[lane][i0 + u] = v[u];
+        else S.blk[i0 + u][lane] = v[u];
+      }
+    }
+    wave_sync();
+    float acc = lane < bs ? S.xs[r0 + lane] : 0.f;
+#pragma unroll 1
+    for (int t = 0; t < bs; t++) {
+      const int c = LOWER ? t : bs - 1 - t;
+      float xc = readlane_f(acc, c);
+      if (!UNIT) xc = xc / guard_f(S.blk[c][c]);
+      if (lane == c) acc = xc;
+      else if (LOWER ? lane > c : lane < c) acc -= S.blk[lane][c] * xc;
+    }
+    if (lane < bs) S.xs[r0 + lane] = acc;
+    wave_sync();
+    {
+      float part[Q];
+      bool todo[Q];
+#pragma unroll
+      for (int s = 0; s < Q; s++) {
+        const int row = lane + 64 * s;
+        todo[s] = row < n && (LOWER ? row >= r0 + bs : row < r0);
+        part[s] = 0.f;
+      }
+#pragma unroll 1
+      for (int c0 = 0; c0 < bs; c0 += 16) {
+        float fv[Q][16];
+#pragma unroll
+        for (int s = 0; s < Q; s++) {
+          const int row = lane + 64 * s;
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int c = c0 + u;
+            const uint32_t off = TRANS ? (uint32_t)(r0 + c) * (uint32_t)n + (uint32_t)row
+                                       : (uint32_t)row * (uint32_t)n + (uint32_t)(r0 + c);
+            fv[s][u] = buf_load<float>(rf, (todo[s] & (c < bs)) ? 4u * off : kBufOob);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const float xc = S.xs[r0 + (c0 + u < bs ? c0 + u : 0)];
+#pragma unroll
+          for (int s = 0; s < Q; s++) part[s] += fv[s][u] * xc;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < Q; s++)
+        if (todo[s]) S.xs[lane + 64 * s] -= part[s];
+    }
+    wave_sync();
+  }
+}
+
 // op(F) x = xs in place, 64 rows at a time: the diagonal block with every
 // index static (values exchanged through scalar reads), then the other rows
 // by one dot product per lane and slot.  TRANS reads F transposed (L^T).
 template <int Q, bool LOWER, bool UNIT, bool TRANS>
-__device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, WaveLds& S, int lane) {
+__device__ __forceinline__ void tri_solve_unrolled(const float* __restrict__ F, int n, WaveLds& S, int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   const int nbk = (n + 63) >> 6;
 #pragma unroll 1
@@ -522,14 +859,30 @@ __device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, Wa
   }
 }
 
-template <int Q, bool CHOL>
-__device__ __forceinline__ void solve_wave(const float* __restrict__ F, int n, WaveLds& S, int lane) {
-  if (CHOL) {
-    tri_solve<Q, true, false, false>(F, n, S, lane);
-    tri_solve<Q, false, false, true>(F, n, S, lane);
+// The classes with n <= 128 run the rolled phases (compact code, so the
+// instruction cache holds a wavefront's loop); the classes with n > 128 run
+// the fully unrolled ones.
+template <int Q>
+constexpr bool kRolled = Q <= 2;
+
+template <int Q, bool CHOL, class L>
+__device__ __forceinline__ void solve_wave(const float* __restrict__ F, int n, L& S, int lane) {
+  if constexpr (kRolled<Q>) {
+    if (CHOL) {
+      tri_solve<Q, true, false, false>(F, n, S, lane);
+      tri_solve<Q, false, false, true>(F, n, S, lane);
+    } else {
+      tri_solve<Q, true, true, false>(F, n, S, lane);
+      tri_solve<Q, false, false, false>(F, n, S, lane);
+    }
   } else {
-    tri_solve<Q, true, true, false>(F, n, S, lane);
-    tri_solve<Q, false, false, false>(F, n, S, lane);
+    if (CHOL) {
+      tri_solve_unrolled<Q, true, false, false>(F, n, S, lane);
+      tri_solve_unrolled<Q, false, false, true>(F, n, S, lane);
+    } else {
+      tri_solve_unrolled<Q, true, true, false>(F, n, S, lane);
+      tri_solve_unrolled<Q, false, false, false>(F, n, S, lane);
+    }
   }
 }
 
@@ -579,11 +932,12 @@ __device__ __forceinline__ double residual_wave(const double* __restrict__ A, co
 
 template <int Q, bool CHOL>
 __global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs args, int cls) {
-  __shared__ WaveLds lds[kWaves];
+  using Lds = typename std::conditional<kRolled<Q>, WaveLdsBlk, WaveLds>::type;
+  __shared__ Lds lds[kWaves];
   const int lane = threadIdx.x & 63;
   // the wavefront index as a scalar, so every LDS address below is one base
   // register plus an immediate offset
-  WaveLds& S = lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+  Lds& S = lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
   const int begin = args.cls[cls], count = args.cls[kClasses + cls];
   for (;;) {
     int slot = 0;
@@ -636,12 +990,17 @@ __global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs arg
     for (int k = 0; k < n; k += kNB) {
       const int nb = min(kNB, n - k);
       if (CHOL) {
-        panel_chol<Q>(F, n, k, nb, S, lane, info);
+        if constexpr (kRolled<Q>) panel_chol<Q>(F, n, k, nb, S, lane, info);
+        else panel_chol_unrolled<Q>(F, n, k, nb, S, lane, info);
       } else {
-        panel_lu<Q>(F, n, k, nb, S, lane, info);
+        if constexpr (kRolled<Q>) panel_lu<Q>(F, n, k, nb, S, lane, info);
+        else panel_lu_unrolled<Q>(F, n, k, nb, S, lane, info);
         wave_sync();
         swap_outside<Q>(F, n, k, nb, S, lane);
-        if (k + nb < n) trsm_u12<Q>(F, n, k, nb, lane);
+        if (k + nb < n) {
+          if constexpr (kRolled<Q>) trsm_u12<Q>(F, n, k, nb, lane);
+          else trsm_u12_unrolled<Q>(F, n, k, nb, lane);
+        }
       }
       trailing_update<CHOL>(F, n, k, nb, lane);
     }
