@@ -35,6 +35,19 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
         "mp_panel": ("hbm", None), "potrf_diag": ("hbm", None), "other": ("hbm", None)}
 
 
+# HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, both factors from a
+# 1 GiB copy kernel) of the kernel classes a roofline object names, measured by
+# rocprofv3 --pmc on numax_amd/lib/nmx_profile at this benchmark's sizes:
+# n = 16384 for the LU paths, the 20,000-system batch for the batched leg.
+# Keyed by (kernel class, n or batch count); another size reports null.
+MEASURED_TRAFFIC = {
+    ("getf2_leaf", 16384): (2.11 + 9.86) * 2 ** 20,        # mixed-precision path, FP32 leaf
+    ("mp_gemm", 16384): (598.59 + 338.67) * 2 ** 20,
+    ("batched_solve", 20000): (11364.06 + 2317.26 + 7290.70 + 1534.23 + 5119.97 + 1010.50 + 3374.91 + 708.27
+                               + 1341.29 + 295.09 + 973.99 + 214.54 + 86.90 + 23.30 + 54.36 + 21.28) * 2 ** 20,
+}
+
+
 def profile_step(ctx, step) -> dict:
     """Per-kernel-class HIP-event times of one step on the library's stream."""
     ctx.set_profiling(True)
@@ -188,7 +201,8 @@ def batched_leg(args, torch) -> dict:
                        f"half Cholesky (G G^T / n + I), FP32 factor + FP64 refinement to {cfg.tol:g}",
            "ms_per_step": dt * 1e3, "solves_per_s": len(n) / dt, "gflops": flops / dt / 1e9,
            "roofline": {"bound": "hbm", "achieved": bytes_alg / kern_s / 1e9, "peak": PEAK["hbm_GBs"],
-                        "unit": "GB/s", "frac": bytes_alg / kern_s / 1e9 / PEAK["hbm_GBs"], "traffic": None,
+                        "unit": "GB/s", "frac": bytes_alg / kern_s / 1e9 / PEAK["hbm_GBs"],
+                        "traffic": MEASURED_TRAFFIC.get(("batched_solve", len(n))),
                         "bytes_per_step": bytes_alg, "kernel_ms": rec["ms"]},
            "max_backward_error": float(np.nanmax(be)), "max_iterations": int(it.max()),
            "failed": int((inf != 0).sum()), "kernel_profile": prof}
@@ -323,7 +337,7 @@ def main() -> int:
             peak = PEAK["hbm_GBs"]
             unit = "GB/s"
         return {"kernel": name, "path": label, "bound": bound[0], "achieved": ach, "peak": peak, "unit": unit,
-                "frac": ach / peak, "traffic": None, "launches_per_step": rec["launches"],
+                "frac": ach / peak, "traffic": MEASURED_TRAFFIC.get((name, n)), "launches_per_step": rec["launches"],
                 "avg_launch_ms": rec["ms"] / max(rec["launches"], 1), "share_of_step": None}
 
     roof = roofline_of(prof_mp, "mp-ir")

@@ -248,15 +248,28 @@ __global__ void __launch_bounds__(256) getf2_nopiv_kernel(float* __restrict__ A,
 // (hi [, lo]) copies.  L21 stays m x k (row-major); U12 (k x m) is stored
 // transposed (m x k) so both MFMA operands read 8 consecutive K values.
 // ---------------------------------------------------------------------------
-__global__ void absmax_kernel(int rows, int cols, const float* __restrict__ X, int ldx, unsigned* __restrict__ out) {
+// max |X| over a rows x cols block (ld ldx) into *out (as float bits; NaN
+// never raises it): rows strided over the workgroups, columns over the
+// threads (coalesced), one atomic per workgroup.
+constexpr int kAbsmaxBlocks = 256;
+__global__ void __launch_bounds__(256) absmax_kernel(int rows, int cols, const float* __restrict__ X, int ldx,
+                                                     unsigned* __restrict__ out) {
+  __shared__ float part[4];
   float m = 0.f;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    float v = fabsf(X[(int64_t)(idx / cols) * ldx + idx % cols]);
-    if (v > m) m = v;  // NaN never raises the scale
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const float* row = X + (int64_t)r * ldx;
+    for (int c = threadIdx.x; c < cols; c += 256) {
+      const float v = fabsf(row[c]);
+      if (v > m) m = v;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+    atomicMax(out, __float_as_uint(m));
+  }
 }
 
 __device__ __forceinline__ float pow2_scale(unsigned maxbits) {
@@ -867,8 +880,8 @@ static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   lt* Bl = base + 3 * op;
   zero_u32k<<<1, 64, 0, ctx->stream>>>(f.maxbits, 2);
   const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
-  absmax_kernel<<<g1, 256, 0, ctx->stream>>>(m, nb, L21, n, f.maxbits);
-  absmax_kernel<<<g1, 256, 0, ctx->stream>>>(nb, m, U12, n, f.maxbits + 1);
+  absmax_kernel<<<std::min(m, kAbsmaxBlocks), 256, 0, ctx->stream>>>(m, nb, L21, n, f.maxbits);
+  absmax_kernel<<<std::min(nb, kAbsmaxBlocks), 256, 0, ctx->stream>>>(nb, m, U12, n, f.maxbits + 1);
   convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, L21, n, 0, f.maxbits, Ah, Al);
   convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, U12, n, 1, f.maxbits + 1, Bh, Bl);
   NMX_HIP(ctx, hipGetLastError());

@@ -464,60 +464,78 @@ __global__ void __launch_bounds__(256) gather_rows_kernel(int n, int ncols, cons
   out[(int64_t)i * ldo + c] = (To)in[(int64_t)perm[i] * ldi + c];
 }
 
-// Base triangular solve for n <= 64: one thread per vector, the triangle and
-// 64 vectors staged in LDS.  Each step is trsm_diag's substitution
-// (panel.mojo:2221-2276): total - sum of products taken in ascending p, then
-// the division unless `unit`.  VEC_ROWS picks where the vectors live: the
-// columns of B (left solves) or its rows (the Cholesky panel X L^-T).
-template <typename T, int UPPER, int UNIT, int TRANS, int VEC_ROWS>
+// Base triangular solve for n <= NB (32 or 64): one thread per vector, the
+// vector in registers, op(T) staged in LDS by columns.  Right-looking with
+// every index static: x_j is finished, then x_i -= op(T)[i][j] x_j for the
+// rows after it -- independent updates, so the substitution is throughput-
+// rather than latency-bound.  A lower solve takes each row's products in
+// ascending column order, as trsm_diag (panel.mojo:2221-2276): running
+// subtraction from the right-hand side, then the division unless `unit`; an
+// upper solve takes them in descending order.  VEC_ROWS picks where the
+// vectors live: the columns of B (left solves) or its rows (the Cholesky
+// panel X L^-T).
+template <typename T, int NB, int UPPER, int UNIT, int TRANS, int VEC_ROWS>
 __global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __restrict__ Tm, int ldt, T* __restrict__ B,
                                                 int ldb) {
-  __shared__ T Ts[64][65];
-  __shared__ T Xs[64][65];
+  __shared__ T Tc[NB][NB + 16 / sizeof(T)];  // Tc[j][i] = op(T)[i][j]
   const int t = threadIdx.x;
+  const int v = blockIdx.x * 64 + t;
+  // padded to NB with the identity, so the sweeps below need no bounds
+  for (int idx = t; idx < NB * NB; idx += 64) {
+    const int r = idx / NB, c = idx % NB;
+    const T val = (r < n && c < n) ? Tm[(int64_t)r * ldt + c] : T(r == c ? 1 : 0);  // T[r][c], read along rows
+    if (TRANS) Tc[r][c] = val;  // op(T)[c][r]
+    else Tc[c][r] = val;        // op(T)[r][c]
+  }
+  __syncthreads();
+  // the block's 64 vectors through one descriptor (32-bit offsets, past-the-
+  // end lanes and rows read 0 and store nothing)
   const int v0 = blockIdx.x * 64;
-  for (int idx = t; idx < n * n; idx += 64) {
-    int i = idx / n, q = idx % n;
-    Ts[i][q] = TRANS ? Tm[(int64_t)q * ldt + i] : Tm[(int64_t)i * ldt + q];
+  const __amdgpu_buffer_rsrc_t rb = buf_rsrc(VEC_ROWS ? B + (int64_t)v0 * ldb : B + v0);
+  const bool act = v < nvec;
+  T x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    const uint32_t off = VEC_ROWS ? (uint32_t)t * (uint32_t)ldb + (uint32_t)i : (uint32_t)i * (uint32_t)ldb + (uint32_t)t;
+    x[i] = buf_load<T>(rb, (act & (i < n)) ? (uint32_t)sizeof(T) * off : kBufOob);
   }
-  if (VEC_ROWS) {
-    for (int idx = t; idx < n * 64; idx += 64) {
-      int v = idx / n, i = idx % n;
-      if (v0 + v < nvec) Xs[i][v] = B[(int64_t)(v0 + v) * ldb + i];
+  // rows and columns past n are the identity and x is 0 there: they change
+  // nothing.  Each step's column is addressed through an index made opaque
+  // after x_j is known, which keeps its LDS reads in that step and after the
+  // division (hoisted, they would hold up to NB^2/2 registers).
+  if (!UPPER) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
+      x[j] = xj;
+      int jj = j;
+      asm volatile("" : "+s"(jj) : "v"(xj));
+      const T* col = Tc[jj];
+#pragma unroll
+      for (int i = j + 1; i < NB; i++) x[i] = x[i] - col[i] * xj;
     }
   } else {
-    for (int idx = t; idx < n * 64; idx += 64) {
-      int i = idx / 64, v = idx % 64;
-      if (v0 + v < nvec) Xs[i][v] = B[(int64_t)i * ldb + v0 + v];
+#pragma unroll
+    for (int j = NB - 1; j >= 0; j--) {
+      const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
+      x[j] = xj;
+      int jj = j;
+      asm volatile("" : "+s"(jj) : "v"(xj));
+      const T* col = Tc[jj];
+#pragma unroll
+      for (int i = 0; i < j; i++) x[i] = x[i] - col[i] * xj;
     }
   }
-  __syncthreads();
-  if (v0 + t < nvec) {
-    if (!UPPER) {
-      for (int i = 0; i < n; i++) {
-        T s = Xs[i][t];
-        for (int q = 0; q < i; q++) s = s - Ts[i][q] * Xs[q][t];
-        Xs[i][t] = UNIT ? s : s / Ts[i][i];
-      }
-    } else {
-      for (int i = n - 1; i >= 0; i--) {
-        T s = Xs[i][t];
-        for (int q = i + 1; q < n; q++) s = s - Ts[i][q] * Xs[q][t];
-        Xs[i][t] = UNIT ? s : s / Ts[i][i];
-      }
-    }
-  }
-  __syncthreads();
-  if (VEC_ROWS) {
-    for (int idx = t; idx < n * 64; idx += 64) {
-      int v = idx / n, i = idx % n;
-      if (v0 + v < nvec) B[(int64_t)(v0 + v) * ldb + i] = Xs[i][v];
-    }
-  } else {
-    for (int idx = t; idx < n * 64; idx += 64) {
-      int i = idx / 64, v = idx % 64;
-      if (v0 + v < nvec) B[(int64_t)i * ldb + v0 + v] = Xs[i][v];
-    }
+  // the store offsets are recomputed from an opaque lane index: shared with
+  // the loads' they would stay live (NB registers) through the sweep
+  int ts = t;
+  asm volatile("" : "+v"(ts));
+  const bool acts = ts + v0 < nvec;
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    const uint32_t off =
+        VEC_ROWS ? (uint32_t)ts * (uint32_t)ldb + (uint32_t)i : (uint32_t)i * (uint32_t)ldb + (uint32_t)ts;
+    buf_store(rb, (acts & (i < n)) ? (uint32_t)sizeof(T) * off : kBufOob, x[i]);
   }
 }
 
@@ -673,7 +691,12 @@ void launch_base(nmx_ctx* ctx, int n, int nvec, const T* Tm, int ldt, T* B, int 
     return;
   }
   ProfScope ps(ctx, K_TRSM, (double)n * n * nvec, sizeof(T) * ((double)n * n + 2.0 * n * nvec));
-  trsm_base<T, UPPER, UNIT, TRANS, VEC_ROWS><<<ceil_div(nvec, 64), 64, 0, ctx->stream>>>(n, nvec, Tm, ldt, B, ldb);
+  if (n <= 32)
+    trsm_base<T, 32, UPPER, UNIT, TRANS, VEC_ROWS><<<ceil_div(nvec, 64), 64, 0, ctx->stream>>>(n, nvec, Tm, ldt, B,
+                                                                                             ldb);
+  else
+    trsm_base<T, 64, UPPER, UNIT, TRANS, VEC_ROWS><<<ceil_div(nvec, 64), 64, 0, ctx->stream>>>(n, nvec, Tm, ldt, B,
+                                                                                             ldb);
 }
 
 template <typename T, int VEC_ROWS>
