@@ -53,7 +53,7 @@ int nmx_version(void);
 /* Optional per-kernel-class event timing on the context's stream (no
  * reference counterpart: it serves the roofline report).  Turning it on or
  * off clears the totals; nmx_ctx_profile synchronizes and returns the totals
- * of class `kind` (0 .. 12, names from nmx_kernel_name). */
+ * of class `kind` (0 .. 13, names from nmx_kernel_name). */
 int nmx_ctx_set_profiling(nmx_ctx* ctx, int on);
 int nmx_ctx_profile(nmx_ctx* ctx, int kind, int64_t* launches, double* ms, double* flops, double* bytes);
 const char* nmx_kernel_name(int kind);
@@ -195,6 +195,29 @@ int nmx_batched_syevj_f32(nmx_ctx* ctx, int count, const int32_t* n, const int64
  * `count` problems A_p (m x n row-major, contiguous, 1 <= n <= 32, m >= n),
  * b_p (m) and x_p (n), FP32, packed back to back. */
 int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const float* A, const float* B, float* X);
+
+/* ---- Householder QR (FP64, m >= n, row-major) --------------------------------
+ * qr_factor (numax/linalg/qr.mojo:615-743): blocked right-looking geqrf in
+ * place -- R on and above the diagonal, the reflectors' V below it (unit
+ * diagonal implicit), tau[n] their scales -- with geqr2_panel's dlarfg
+ * convention (numax/linalg/panel.mojo:1975-2073) per `block`-wide panel
+ * (1..64; numax's default is 16) and the panel's block reflector applied
+ * to the trailing columns as C -= V (T^T (V^T C)) (qr.mojo:123-327). */
+int nmx_geqrf_f64(nmx_ctx* ctx, int m, int n, double* A, int lda, double* tau, int block);
+/* QR.apply_q_transpose (qr.mojo:470-534) when trans != 0: B := Q^T B for an
+ * m x nrhs B; trans == 0 applies Q. */
+int nmx_ormqr_f64(nmx_ctx* ctx, int trans, int m, int n, int nrhs, const double* QR, int lda, const double* tau,
+                  double* B, int ldb, int block);
+/* QR.q (qr.mojo:405-468): the thin m x n Q, the panels applied to the
+ * identity last first. */
+int nmx_orgqr_f64(nmx_ctx* ctx, int m, int n, const double* QR, int lda, const double* tau, double* Q, int ldq,
+                  int block);
+/* QR.r (qr.mojo:371-403): the n x n upper triangle, zeros below. */
+int nmx_qr_r_f64(nmx_ctx* ctx, int n, const double* QR, int lda, double* R, int ldr);
+/* lstsq(method "qr") and QR.solve (qr.mojo:746-821, :536-612): factors A in
+ * place (tau[n]), then B (m x nrhs) := Q^T B and R X = B[0:n] solved in
+ * place -- X is B's first n rows. */
+int nmx_gels_f64(nmx_ctx* ctx, int m, int n, int nrhs, double* A, int lda, double* tau, double* B, int ldb, int block);
 
 /* ---- synthetic inputs ------------------------------------------------------
  * Philox4x32-10 stream (the same words as numax_amd.random): kind 0 fills

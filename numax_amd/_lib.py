@@ -47,6 +47,11 @@ SIGNATURES = {
     "nmx_potrf_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_potrs_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_potrs_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_geqrf_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_ormqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
+    "nmx_orgqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
+    "nmx_qr_r_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_gels_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_gemm_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_vp, _c_int, _c_vp,
                               _c_int, _c_dbl, _c_vp, _c_int]),
     "nmx_gemm_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_flt, _c_vp, _c_int, _c_vp,
@@ -149,7 +154,7 @@ class Context:
         """Per kernel class: launches, milliseconds, algorithmic flops and bytes."""
         lib = load()
         out = {}
-        for kind in range(13):
+        for kind in range(14):
             n = _c_i64()
             ms, fl, by = _c_dbl(), _c_dbl(), _c_dbl()
             check(lib.nmx_ctx_profile(self.handle, kind, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl),

@@ -266,6 +266,6 @@ NMX_API int nmx_ctx_profile(nmx_ctx* c, int kind, int64_t* launches, double* ms,
 NMX_API const char* nmx_kernel_name(int kind) {
   static const char* names[K_COUNT] = {"gemm",       "getf2_leaf", "laswp",    "trsm_base", "potrf_diag",
                                        "mp_gemm",    "mp_panel",   "residual", "mp_trsv",   "other",
-                                       "batched_solve", "batched_eigh", "batched_lstsq"};
+                                       "batched_solve", "batched_eigh", "batched_lstsq", "qr_panel"};
   return (kind >= 0 && kind < K_COUNT) ? names[kind] : "unknown";
 }

@@ -48,6 +48,7 @@ enum NmxKernel {
   K_BATCH,        // batched uneven-size solves (all size classes of one call)
   K_EIGH,         // batched Jacobi eigensolver
   K_TSQR,         // batched tall-skinny least squares
+  K_QR_PANEL,     // Householder panel (geqr2)
   K_COUNT
 };
 

@@ -234,6 +234,107 @@ def solve_triangular(a, b, upper: bool = False, unit: bool = False, trans: bool 
     return x
 
 
+class QR:
+    """``QR`` (numax/linalg/qr.mojo:330-612): a blocked Householder QR held on
+    the device -- ``factored`` has R on and above the diagonal and the
+    reflectors below it (unit diagonal implicit), ``taus`` one scale per
+    column, ``block`` the panel width it ran at.  ``r()`` and ``q()`` form
+    the factors; ``apply_q_transpose`` and ``solve`` never form Q."""
+
+    def __init__(self, factored: torch.Tensor, taus: torch.Tensor, block: int):
+        self.factored = factored
+        self.taus = taus
+        self.block = int(block)
+
+    @property
+    def m(self) -> int:
+        return int(self.factored.shape[0])
+
+    @property
+    def n(self) -> int:
+        return int(self.factored.shape[1])
+
+    def r(self) -> torch.Tensor:
+        """The ``n x n`` upper-triangular R, zeros below (qr.mojo:371-403)."""
+        n = self.n
+        out = torch.empty((n, n), dtype=torch.float64, device=self.factored.device)
+        _lib.check(_lib.load().nmx_qr_r_f64(_ctx(out).handle, n, _ptr(self.factored), max(n, 1), _ptr(out),
+                                            max(n, 1)), "QR.r")
+        return out
+
+    def q(self) -> torch.Tensor:
+        """The thin ``m x n`` Q with orthonormal columns (qr.mojo:405-468)."""
+        m, n = self.m, self.n
+        out = torch.empty((m, n), dtype=torch.float64, device=self.factored.device)
+        _lib.check(_lib.load().nmx_orgqr_f64(_ctx(out).handle, m, n, _ptr(self.factored), max(n, 1),
+                                             _ptr(self.taus), _ptr(out), max(n, 1), self.block), "QR.q")
+        return out
+
+    def _apply(self, b, trans: int, who: str) -> torch.Tensor:
+        x = _device_tensor(b, torch.float64).clone()
+        if x.shape[0] != self.m:
+            raise NumaxError(f"{who}: b has {x.shape[0]} rows for an m = {self.m} factorization")
+        nrhs = 1 if x.dim() == 1 else int(x.shape[1])
+        _lib.check(_lib.load().nmx_ormqr_f64(_ctx(x).handle, trans, self.m, self.n, nrhs, _ptr(self.factored),
+                                             max(self.n, 1), _ptr(self.taus), _ptr(x), max(nrhs, 1), self.block),
+                   who)
+        return x
+
+    def apply_q_transpose(self, b) -> torch.Tensor:
+        """``Q^T b`` for an ``m``-row ``b`` (qr.mojo:470-534)."""
+        return self._apply(b, 1, "QR.apply_q_transpose")
+
+    def solve(self, b) -> torch.Tensor:
+        """Least-squares ``x`` of ``A x ~= b`` (qr.mojo:536-612): ``Q^T b``, then
+        the back substitution with R; Q is never formed."""
+        y = self._apply(b, 1, "QR.solve")
+        n = self.n
+        nrhs = 1 if y.dim() == 1 else int(y.shape[1])
+        x = y[:n].clone()
+        _lib.check(_lib.load().nmx_trsm_f64(_ctx(x).handle, 1, 0, 0, n, nrhs, _ptr(self.factored), max(n, 1),
+                                            _ptr(x), max(nrhs, 1)), "QR.solve")
+        return x
+
+
+def _qr_input(a, who: str) -> torch.Tensor:
+    t = _device_tensor(a)
+    if t.dtype != torch.float64:
+        raise NumaxError(f"{who}: dtype {t.dtype} is not supported by the device QR (float64)")
+    if t.dim() != 2 or t.shape[0] < t.shape[1]:
+        raise NumaxError(f"{who}: needs an m x n matrix with m >= n, got shape {tuple(t.shape)}")
+    return t
+
+
+def qr_factor(a, block: int = 16) -> QR:
+    """``qr_factor`` (numax/linalg/qr.mojo:615-743): blocked Householder QR of
+    an ``m x n`` float64 matrix with ``m >= n``; ``block`` is the panel width
+    (1..64, numax's default 16)."""
+    work = _qr_input(a, "qr_factor").clone()
+    m, n = work.shape
+    taus = torch.zeros(n, dtype=torch.float64, device=work.device)
+    _lib.check(_lib.load().nmx_geqrf_f64(_ctx(work).handle, m, n, _ptr(work), max(n, 1), _ptr(taus), block),
+               "qr_factor")
+    return QR(work, taus, block)
+
+
+def lstsq(a, b, block: int = 16, method: str = "qr") -> torch.Tensor:
+    """``lstsq`` (numax/linalg/qr.mojo:746-821), method "qr": the ``x``
+    minimizing ``||A x - b||`` for ``m >= n`` -- ``qr_factor`` then
+    ``QR.solve`` in one call."""
+    if method != "qr":
+        raise NumaxError(f"lstsq: method {method!r} is not on this device path (\"qr\")")
+    work = _qr_input(a, "lstsq").clone()
+    m, n = work.shape
+    x = _device_tensor(b, torch.float64).clone()
+    if x.shape[0] != m:
+        raise NumaxError(f"lstsq: b has {x.shape[0]} rows for an {m} x {n} matrix")
+    nrhs = 1 if x.dim() == 1 else int(x.shape[1])
+    taus = torch.zeros(n, dtype=torch.float64, device=work.device)
+    _lib.check(_lib.load().nmx_gels_f64(_ctx(work).handle, m, n, nrhs, _ptr(work), max(n, 1), _ptr(taus), _ptr(x),
+                                        max(nrhs, 1), block), "lstsq")
+    return x[:n].clone()
+
+
 def gemm(a, b, c=None, alpha: float = 1.0, beta: float = 0.0, transa: bool = False, transb: bool = False):
     """``c = beta c + alpha op(a) op(b)`` on the MFMA kernel the drivers use."""
     A = _device_tensor(a)
