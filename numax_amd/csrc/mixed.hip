@@ -13,11 +13,18 @@
 //          update is bound by its FP32 C traffic.
 // Refine:  r = b - A x in FP64 on the original matrix, d = V U^-1 L^-1 W^T r
 //          in FP32, x += d in FP64, until the normwise backward error meets
-//          the target.
+//          the target.  Past the point where the low-precision factor stops
+//          contracting the error (kappa u_f ~ 1, kappa ~ 1e4 for FP16), the
+//          correction equation A d = r is solved instead by GMRES in
+//          FP64, left-preconditioned by the same factor (GMRES-IR, Carson and
+//          Higham, SIAM J. Sci. Comput. 40(2), 2018): the Krylov basis,
+//          the Arnoldi products (classical Gram-Schmidt, twice) and the
+//          matrix products are FP64; the preconditioner solves are FP32.
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "nmx_internal.h"
 
@@ -684,48 +691,63 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
 }
 
 // ---------------------------------------------------------------------------
-// FP64 residual r = b - A x (one wavefront per row), and the norms the
-// stopping test needs: max |r|, max |x| per column as float bits, and on the
-// first pass the row sums of |A| (max taken on the host).
-// ---------------------------------------------------------------------------
+// r = b - A x (b == nullptr: r = -A x) in FP64, one wavefront per row, the
+// rows strided over at most kResidualBlocks workgroups; with `norms`, the
+// max |r|, max |x| and (with rowabs) max row sum |A| of the stopping test,
+// one atomic per workgroup each.
+constexpr int kResidualBlocks = 1024;
 __global__ void __launch_bounds__(256) residual_kernel(int n, const double* __restrict__ A, int lda,
                                                        const double* __restrict__ x, const double* __restrict__ b,
                                                        double* __restrict__ r, double* __restrict__ rowabs,
                                                        unsigned long long* __restrict__ norms) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)row * lda);
+  __shared__ unsigned long long red[3][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x);
-  double s = 0.0, sa = 0.0;
-  // 8 column chunks of 64 in flight per step; past-the-end reads are 0
-  for (int c0 = 0; c0 < n; c0 += 512) {
-    double av[8], xv[8];
+  // running maxima as bit patterns of non-negative doubles (a NaN stays the max)
+  unsigned long long mr = 0, mx = 0, ma = 0;
+  for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
+    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)row * lda);
+    double s = 0.0, sa = 0.0;
+    // 8 column chunks of 64 in flight per step; past-the-end reads are 0
+    for (int c0 = 0; c0 < n; c0 += 512) {
+      double av[8], xv[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int c = c0 + u * 64 + lane;
-      const uint32_t off = c < n ? (uint32_t)c * 8u : kBufOob;
-      av[u] = buf_load<double>(ra, off);
-      xv[u] = buf_load<double>(rx, off);
+      for (int u = 0; u < 8; u++) {
+        const int c = c0 + u * 64 + lane;
+        const uint32_t off = c < n ? (uint32_t)c * 8u : kBufOob;
+        av[u] = buf_load<double>(ra, off);
+        xv[u] = buf_load<double>(rx, off);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        s += av[u] * xv[u];
+        sa += fabs(av[u]);
+      }
     }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      s += av[u] * xv[u];
-      sa += fabs(av[u]);
+    for (int off = 32; off > 0; off >>= 1) {
+      s += __shfl_xor(s, off);
+      sa += __shfl_xor(sa, off);
+    }
+    if (lane == 0) {
+      const double rv = (b ? b[row] : 0.0) - s;
+      r[row] = rv;
+      if (rowabs) rowabs[row] = sa;
+      mr = max(mr, (unsigned long long)__double_as_longlong(fabs(rv)));
+      mx = max(mx, (unsigned long long)__double_as_longlong(fabs(x[row])));
+      ma = max(ma, (unsigned long long)__double_as_longlong(sa));
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    s += __shfl_xor(s, off);
-    sa += __shfl_xor(sa, off);
-  }
+  if (!norms) return;  // uniform: the whole grid takes the same branch
   if (lane == 0) {
-    const double rv = b[row] - s;
-    r[row] = rv;
-    if (rowabs) rowabs[row] = sa;
-    const double ar = fabs(rv), ax = fabs(x[row]);
-    atomicMax(&norms[0], (unsigned long long)__double_as_longlong(ar));
-    atomicMax(&norms[1], (unsigned long long)__double_as_longlong(ax));
-    if (rowabs) atomicMax(&norms[2], (unsigned long long)__double_as_longlong(sa));
+    red[0][wv] = mr;
+    red[1][wv] = mx;
+    red[2][wv] = ma;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    const unsigned long long v = max(max(red[q][0], red[q][1]), max(red[q][2], red[q][3]));
+    if (q < 2 || rowabs) atomicMax(&norms[q], v);
   }
 }
 
@@ -987,6 +1009,41 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
   return NMX_OK;
 }
 
+// ---- GMRES-IR building blocks (FP64) ------------------------------------------
+// out[i] = V_i . w for the k basis vectors V_i (rows of V, length n, ld n).
+constexpr int kGmDotThreads = 256;
+__global__ void __launch_bounds__(kGmDotThreads) gm_dots_kernel(int n, const double* __restrict__ V,
+                                                                const double* __restrict__ w, double* __restrict__ out) {
+  __shared__ double red[kGmDotThreads / 64];
+  const double* v = V + (int64_t)blockIdx.x * n;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n; e += kGmDotThreads) s += v[e] * w[e];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kGmDotThreads / 64; q++) t += red[q];
+    out[blockIdx.x] = t;
+  }
+}
+
+// w -= sum_{i<k} h_i V_i  (h on the device)
+__global__ void gm_update_kernel(int n, int k, const double* __restrict__ V, const double* __restrict__ h,
+                                 double* __restrict__ w) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = w[e];
+  for (int i = 0; i < k; i++) s -= h[i] * V[(int64_t)i * n + e];
+  w[e] = s;
+}
+
+// v = alpha w
+__global__ void gm_scale_kernel(int n, double alpha, const double* __restrict__ w, double* __restrict__ v) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) v[e] = alpha * w[e];
+}
+
 // d (FP64, n) = V U^-1 L^-1 W^T r  (accumulate: x += d)
 static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, float* tmp, int accumulate) {
   const int n = f.n, n4 = n & ~3, h = n4 / 4;
@@ -1022,13 +1079,109 @@ static int residual(nmx_ctx* ctx, int n, const double* A, int lda, const double*
   zero_u64<<<1, 64, 0, ctx->stream>>>(norms, 3);
   {
     ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n + 24.0 * n);
-    residual_kernel<<<ceil_div(n, 4), 256, 0, ctx->stream>>>(n, A, lda, x, b, r, rowabs, norms);
+    residual_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda, x, b, r, rowabs,
+                                                                                       norms);
     NMX_HIP(ctx, hipGetLastError());
   }
   unsigned long long h[3];
   NMX_HIP(ctx, hipMemcpyAsync(h, norms, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < 3; i++) memcpy(&out[i], &h[i], 8);
+  return NMX_OK;
+}
+
+// ---- GMRES-IR correction ------------------------------------------------------
+struct Gmres {
+  double* V;   // (restart + 1) x n Krylov basis, FP64
+  double* w;   // n: -A v_j
+  double* h;   // 3 (restart + 2): two Gram-Schmidt passes and the norm
+  double* y;   // restart: the combination added to x
+  int restart;
+};
+
+static size_t gmres_doubles(int n, int restart) {
+  return (size_t)(restart + 1) * n + n + 3 * (size_t)(restart + 2) + restart;
+}
+
+// x += d with d from GMRES(restart) on M^-1 A d = M^-1 r started at d = 0,
+// stopped when the preconditioned residual estimate falls under tol_in of
+// its start (or the Krylov space is exhausted).  *inner: iterations taken.
+static int gmres_correction(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, const double* r, double* x,
+                            float* tmp, const Gmres& g, double tol_in, int* inner) {
+  const int n = f.n, m = g.restart;
+  const unsigned nbk = (unsigned)ceil_div(n, 256);
+  double* hv = g.h;
+  *inner = 0;
+  NMX_TRY(mp_solve_vec(ctx, f, r, g.V, tmp, 0));  // z = M^-1 r
+  gm_dots_kernel<<<1, kGmDotThreads, 0, ctx->stream>>>(n, g.V, g.V, hv);
+  NMX_HIP(ctx, hipGetLastError());
+  double beta2 = 0.0;
+  NMX_HIP(ctx, hipMemcpyAsync(&beta2, hv, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double beta = sqrt(beta2);
+  if (!(beta > 0.0) || !std::isfinite(beta)) return NMX_OK;
+  gm_scale_kernel<<<nbk, 256, 0, ctx->stream>>>(n, 1.0 / beta, g.V, g.V);
+  NMX_HIP(ctx, hipGetLastError());
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), hh(3 * (m + 2));
+  gv[0] = beta;
+  int j = 0;
+  while (j < m) {
+    const double* vj = g.V + (int64_t)j * n;
+    double* vn = g.V + (int64_t)(j + 1) * n;
+    {
+      ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n);
+      residual_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda, vj, nullptr,
+                                                                                         g.w, nullptr, nullptr);
+      NMX_HIP(ctx, hipGetLastError());
+    }
+    NMX_TRY(mp_solve_vec(ctx, f, g.w, vn, tmp, 0));  // u = -M^-1 A v_j
+    // classical Gram-Schmidt, twice, against v_0 .. v_j; then |u|^2
+    gm_dots_kernel<<<j + 1, kGmDotThreads, 0, ctx->stream>>>(n, g.V, vn, hv);
+    gm_update_kernel<<<nbk, 256, 0, ctx->stream>>>(n, j + 1, g.V, hv, vn);
+    gm_dots_kernel<<<j + 1, kGmDotThreads, 0, ctx->stream>>>(n, g.V, vn, hv + (m + 2));
+    gm_update_kernel<<<nbk, 256, 0, ctx->stream>>>(n, j + 1, g.V, hv + (m + 2), vn);
+    gm_dots_kernel<<<1, kGmDotThreads, 0, ctx->stream>>>(n, vn, vn, hv + 2 * (m + 2));
+    NMX_HIP(ctx, hipGetLastError());
+    NMX_HIP(ctx, hipMemcpyAsync(hh.data(), hv, sizeof(double) * hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+    NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // u held -(M^-1 A v_j), so the Hessenberg column is the negated projections
+    for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = -(hh[i] + hh[(m + 2) + i]);
+    const double hn = sqrt(hh[2 * (m + 2)]);
+    H[(size_t)(j + 1) * m + j] = hn;
+    if (hn > 0.0) {
+      gm_scale_kernel<<<nbk, 256, 0, ctx->stream>>>(n, -1.0 / hn, vn, vn);
+      NMX_HIP(ctx, hipGetLastError());
+    }
+    // earlier rotations, then the one that zeroes H[j+1][j]
+    for (int i = 0; i < j; i++) {
+      const double a = H[(size_t)i * m + j], b2 = H[(size_t)(i + 1) * m + j];
+      H[(size_t)i * m + j] = cs[i] * a + sn[i] * b2;
+      H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b2;
+    }
+    const double a = H[(size_t)j * m + j], b2 = H[(size_t)(j + 1) * m + j];
+    const double rho = hypot(a, b2);
+    cs[j] = rho > 0.0 ? a / rho : 1.0;
+    sn[j] = rho > 0.0 ? b2 / rho : 0.0;
+    H[(size_t)j * m + j] = rho;
+    H[(size_t)(j + 1) * m + j] = 0.0;
+    gv[j + 1] = -sn[j] * gv[j];
+    gv[j] = cs[j] * gv[j];
+    j++;
+    if (!(fabs(gv[j]) > tol_in * beta) || !(hn > 0.0)) break;
+  }
+  // y = H[0:j, 0:j]^-1 g[0:j]; x += V y (as x -= V (-y))
+  std::vector<double> yv(j, 0.0);
+  for (int i = j - 1; i >= 0; i--) {
+    double t = gv[i];
+    for (int c = i + 1; c < j; c++) t -= H[(size_t)i * m + c] * yv[c];
+    yv[i] = H[(size_t)i * m + i] != 0.0 ? t / H[(size_t)i * m + i] : 0.0;
+  }
+  for (double& v : yv) v = -v;
+  NMX_HIP(ctx, hipMemcpyAsync(g.y, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
+  gm_update_kernel<<<nbk, 256, 0, ctx->stream>>>(n, j, g.V, g.y, x);
+  NMX_HIP(ctx, hipGetLastError());
+  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // yv's host memory is released on return
+  *inner = j;
   return NMX_OK;
 }
 
@@ -1050,6 +1203,8 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   if (cfg.precision < 0 || cfg.precision > 3) return -7;
   if (cfg.block <= 0) cfg.block = 256;
   if (cfg.block % 64) return -7;
+  if (cfg.solver < 0 || cfg.solver > 2) return -7;
+  if (cfg.solver != 1 && (cfg.gmres_restart < 1 || cfg.gmres_restart > 100)) return -7;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
   nmx_ir_report local{};
   nmx_ir_report* R = rep ? rep : &local;
@@ -1060,7 +1215,9 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   int st = mp_alloc(ctx, f, n, cfg.block, cfg.precision);
   f.pivot = cfg.pivot;
   void* wsp = nullptr;
-  if (st == NMX_OK) st = nmx_workspace(ctx, sizeof(double) * (3 * (size_t)n + 8) + sizeof(float) * (size_t)n, &wsp);
+  const size_t gm_doubles = cfg.solver == 1 ? 0 : gmres_doubles(n, cfg.gmres_restart);
+  if (st == NMX_OK)
+    st = nmx_workspace(ctx, sizeof(double) * (3 * (size_t)n + 8 + gm_doubles) + sizeof(float) * (size_t)n, &wsp);
   if (st != NMX_OK) {
     mp_free(ctx, f);
     return st;
@@ -1069,7 +1226,15 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   double* rowabs = r + n;
   double* d = rowabs + n;
   unsigned long long* norms = (unsigned long long*)(d + n);
-  float* tmp = (float*)(norms + 8);
+  Gmres gm{};
+  gm.restart = cfg.gmres_restart;
+  if (gm_doubles) {
+    gm.V = (double*)(norms + 8);
+    gm.w = gm.V + (size_t)(gm.restart + 1) * n;
+    gm.h = gm.w + n;
+    gm.y = gm.h + 3 * (size_t)(gm.restart + 2);
+  }
+  float* tmp = (float*)((double*)(norms + 8) + gm_doubles);
 
   hipEvent_t e0, e1, e2;
   hipEventCreate(&e0);
@@ -1086,6 +1251,13 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   double anorm = 0.0;
   int it = 0;
   double eta = INFINITY, prev = INFINITY;
+  // solver 1: classical steps only; 2: GMRES-IR from the first correction;
+  // 0: classical while each step at least halves the backward error,
+  // GMRES-IR from the first step that does not (the preconditioner is
+  // applied in FP32, so where classical steps still contract GMRES costs
+  // more solves for the same accuracy)
+  bool use_gmres = cfg.solver == 2;
+  int inner_total = 0;
   while (st == NMX_OK) {
     st = residual(ctx, n, A, lda, x, b, r, it == 0 ? rowabs : nullptr, norms, nrm);
     if (st != NMX_OK) break;
@@ -1093,11 +1265,23 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
     eta = nrm[0] / (anorm * nrm[1]);
     R->history[it < 32 ? it : 31] = eta;
     if (!(eta > cfg.tol) || it >= cfg.max_iter) break;
-    if (it > 0 && !(eta < 0.5 * prev)) break;  // stagnation
+    if (it > 0 && !(eta < 0.5 * prev)) {  // stagnation
+      if (cfg.solver != 0 || use_gmres) break;
+      use_gmres = true;
+    }
     prev = eta;
-    st = mp_solve_vec(ctx, f, r, x, tmp, 1);
+    if (use_gmres) {
+      // reduce the preconditioned residual by what the target still needs, with margin
+      const double tol_in = std::min(1e-2, std::max(1e-10, 0.1 * cfg.tol / eta));
+      int inner = 0;
+      st = gmres_correction(ctx, f, A, lda, r, x, tmp, gm, tol_in, &inner);
+      inner_total += inner;
+    } else {
+      st = mp_solve_vec(ctx, f, r, x, tmp, 1);
+    }
     it++;
   }
+  R->gmres_iterations = inner_total;
   hipEventRecord(e2, ctx->stream);
   hipEventSynchronize(e2);
   float fms = 0.f, rms = 0.f;

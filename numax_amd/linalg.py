@@ -350,14 +350,22 @@ def gemm(a, b, c=None, alpha: float = 1.0, beta: float = 0.0, transa: bool = Fal
     return C
 
 
+SOLVERS = {"auto": 0, "classical": 1, "gmres": 2}
+
+
 def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block: int = 256,
-               seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial"):
+               seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial", solver: str = "auto",
+               gmres_restart: int = 30):
     """``x`` with ``A x = b`` to FP64 backward error from a low-precision
     factorization (no reference counterpart; numax's ``solve`` is the FP64
     path).  ``precision`` picks the trailing-update arithmetic: ``"fp16"``,
     ``"bf16"``, ``"fp16s"`` (FP16 hi + lo split, ~FP32 accuracy) or ``"fp32"``.
-    Returns ``(x, report)``; ``report`` holds the refinement history and the
-    final normwise backward error."""
+    ``solver``: ``"classical"`` refinement, ``"gmres"`` (GMRES-IR: FP64 GMRES
+    on the correction equation, preconditioned by the low-precision factor)
+    or ``"auto"`` (classical while each step at least halves the backward
+    error, GMRES-IR from the first step that does not).  Returns ``(x, report)``; ``report`` holds the
+    refinement history, the GMRES iterations and the final normwise
+    backward error."""
     A = _device_tensor(a, torch.float64)
     n = _square(A, "solve_mpir")
     B = _device_tensor(b, torch.float64)
@@ -382,13 +390,18 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
     if pivot not in ("partial", "butterfly"):
         raise NumaxError("solve_mpir: pivot must be 'partial' or 'butterfly'")
     cfg.pivot = 1 if pivot == "partial" else 0
+    if solver not in SOLVERS:
+        raise NumaxError(f"solve_mpir: solver must be one of {sorted(SOLVERS)}")
+    cfg.solver = SOLVERS[solver]
+    cfg.gmres_restart = gmres_restart
     ctx = _ctx(A)
     xs, reports = [], []
     for col in cols:
         x = torch.empty_like(col)
         rep = _lib.IrReport()
         st = lib.nmx_gesv_mpir(ctx.handle, n, _ptr(A), n, _ptr(col), _ptr(x), ctypes.byref(cfg), ctypes.byref(rep))
-        info = {"iterations": rep.iterations, "converged": bool(rep.converged), "backward_error": rep.backward_error,
+        info = {"iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations,
+                "converged": bool(rep.converged), "backward_error": rep.backward_error,
                 "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms, "info": rep.info,
                 "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
         if st != 0 and not (st == 5 and not raise_on_failure):

@@ -87,3 +87,54 @@ def test_mpir_matrix_rhs_and_errors():
         nl.solve_mpir(a, np.ones(n - 1))
     with pytest.raises(nl.NumaxError, match="precision"):
         nl.solve_mpir(a, bm[:, 0], precision="fp8")
+
+
+def test_gmres_ir_converges_where_classical_refinement_fails():
+    """Plain FP16 factor at kappa = 1e4 (kappa u_f ~ 5): classical steps do
+    not contract, GMRES-IR (FP64 GMRES preconditioned by the factor) reaches
+    FP64 backward error."""
+    n = 1024
+    a = svd_matrix(n, 1e4, seed=11)
+    b = R.normal(12, (n,))
+    _, rep_c = nl.solve_mpir(a, b, precision="fp16", solver="classical", max_iter=20, raise_on_failure=False)
+    assert not rep_c["converged"]
+    x, rep = nl.solve_mpir(a, b, precision="fp16", solver="gmres", max_iter=20)
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    eta_o, xo = fp64_eta(a, b)
+    assert rep["converged"] and eta <= max(1e-15, 2 * eta_o), (rep, eta, eta_o)
+    assert rep["gmres_iterations"] > 0
+    assert np.max(np.abs(x.cpu().numpy() - xo)) / np.max(np.abs(xo)) < 1e-10
+
+
+@pytest.mark.parametrize("kappa", [1e4, 1e6])
+def test_gmres_ir_split_fp16(kappa):
+    n = 1024
+    a = svd_matrix(n, kappa, seed=11)
+    b = R.normal(12, (n,))
+    x, rep = nl.solve_mpir(a, b, precision="fp16s", solver="gmres", max_iter=20)
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    eta_o, _ = fp64_eta(a, b)
+    assert rep["converged"] and eta <= max(1e-15, 2 * eta_o), (rep, eta, eta_o)
+    assert rep["gmres_iterations"] > 0
+
+
+def test_auto_switches_to_gmres_when_classical_stagnates():
+    n = 1024
+    a = svd_matrix(n, 1e4, seed=11)
+    b = R.normal(12, (n,))
+    x, rep = nl.solve_mpir(a, b, precision="fp16", solver="auto", max_iter=20)
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    assert rep["converged"] and eta <= 1e-15, rep
+    assert rep["gmres_iterations"] > 0
+    # where classical steps contract, auto stays classical
+    _, rep2 = nl.solve_mpir(a, b, precision="fp16s", solver="auto")
+    assert rep2["converged"] and rep2["gmres_iterations"] == 0
+
+
+@pytest.mark.parametrize("n", [7, 257, 1030])
+def test_gmres_ir_odd_sizes(n):
+    a = svd_matrix(n, 1e3, seed=n)
+    b = R.normal(n + 1, (n,))
+    x, rep = nl.solve_mpir(a, b, precision="fp16", solver="gmres", gmres_restart=8, max_iter=30)
+    eta = oracle.backward_error(a, x.cpu().numpy(), b)
+    assert rep["converged"] and eta <= 1e-15, (rep, eta)
