@@ -221,6 +221,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-15)
     ap.add_argument("--all-kappa", action="store_true", help="also time kappa = 1e2, 1e4, 1e6")
     ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--block-rhs", type=int, default=64,
+                    help="also time one factor + refine of this many right-hand sides against plain FP64 (0: skip)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
@@ -324,6 +326,39 @@ def main() -> int:
         a = c3_matrix(n, args.kappa, torch)
     prof_mp = profile_step(ctx, mp_step)
 
+    block_rhs = None
+    if args.block_rhs > 1:
+        k = args.block_rhs
+        Bk = R.device_normal(5, (n, k))
+        Xk = torch.empty_like(Bk)
+        Xp = torch.empty_like(Bk)
+        reps = (_lib.IrReport * k)()
+
+        def mp_block_step():
+            st = lib.nmx_gesv_mpir_nrhs(ctx.handle, n, k, P(a), n, P(Bk), k, P(Xk), k, ctypes.byref(cfg), reps)
+            if st not in (0, 5):
+                _lib.check(st, "nmx_gesv_mpir_nrhs")
+
+        def plain_block_step():
+            work.copy_(a)
+            _lib.check(lib.nmx_getrf_f64(ctx.handle, n, P(work), n, P(piv), P(info), args.block, 32), "getrf")
+            Xp.copy_(Bk)
+            _lib.check(lib.nmx_getrs_f64(ctx.handle, n, k, P(work), n, P(piv), P(Xp), k), "getrs")
+
+        bsteps = max(1, min(args.steps, 3))
+        profile_step(ctx, mp_block_step)
+        dt_pk = timed(plain_block_step, bsteps, 1, f"plain-fp64 nrhs={k}")
+        dt_mk = timed(mp_block_step, bsteps, 1, f"mp-ir nrhs={k}")
+        block_rhs = {"nrhs": k, "kappa": args.kappa,
+                     "mp_ms_per_step": dt_mk * 1e3, "mp_solves_per_s": k / dt_mk,
+                     "plain_ms_per_step": dt_pk * 1e3, "plain_solves_per_s": k / dt_pk,
+                     "speedup_vs_plain_fp64": dt_pk / dt_mk,
+                     "mp_backward_error_max": backward_error(a, Xk, Bk),
+                     "plain_backward_error_max": backward_error(a, Xp, Bk),
+                     "mp_iterations_max": max(reps[j].iterations for j in range(k)),
+                     "all_converged": all(bool(reps[j].converged) for j in range(k))}
+        log("block rhs: " + json.dumps(block_rhs))
+
     def roofline_of(prof, label):
         name, rec = max(prof.items(), key=lambda kv: kv[1]["ms"])
         bound = ROOF.get(name, ("hbm", None))
@@ -380,6 +415,7 @@ def main() -> int:
                        "tflops": flops / dt_plain / 1e12, "backward_error": eta_plain, "nrhs": nrhs,
                        "roofline": roof_plain, "kernel_profile": prof_plain},
         "mp_by_kappa": results,
+        "mp_block_rhs": block_rhs,
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_uneven": batched,

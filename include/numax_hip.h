@@ -141,6 +141,16 @@ typedef struct nmx_ir_report {
 void nmx_ir_cfg_default(nmx_ir_cfg* cfg);
 int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x, const nmx_ir_cfg* cfg,
                   nmx_ir_report* report);
+/* The same solve for an n x nrhs block of right-hand sides (row-major B, X
+ * with leading dimensions ldb, ldx >= nrhs): one factorization, then every
+ * column refined together -- the FP64 residual R = B - A X is one MFMA GEMM
+ * per step, the corrections one pair of blocked FP32 triangular solves --
+ * until each column meets cfg->tol (converged columns get no further
+ * correction).  Classical refinement; `reps` (nrhs entries, may be NULL)
+ * receives a report per column.  No reference counterpart: numax's
+ * solve(a, b) with a matrix b is the FP64 path. */
+int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* B, int ldb, double* X,
+                       int ldx, const nmx_ir_cfg* cfg, nmx_ir_report* reps);
 /* Test hook of the factor stage: stage 0 returns W^T A V, stage 1 its FP32
  * no-pivot L\U; `diags` receives the 4n butterfly diagonals (W's r0, r1, s0,
  * s1, t0, t1 then V's), F is n x n row-major (device), info the first zero

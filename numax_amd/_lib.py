@@ -47,6 +47,8 @@ SIGNATURES = {
     "nmx_potrf_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_potrs_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_potrs_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_gesv_mpir_nrhs": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp,
+                                    _c_vp]),
     "nmx_geqrf_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_ormqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_orgqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),

@@ -1009,6 +1009,58 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
   return NMX_OK;
 }
 
+// ---- multi-RHS refinement building blocks -------------------------------------
+// x (+)= y over an n x nrhs block (row-major, leading dimensions ldy / ldx)
+__global__ void mat_f32_add_f64(int n, int nrhs, const float* __restrict__ y, int ldy, double* __restrict__ x,
+                                int ldx, int accumulate) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * nrhs) return;
+  const int r = (int)(idx / nrhs), c = (int)(idx % nrhs);
+  const double v = (double)y[(int64_t)r * ldy + c];
+  double* px = x + (int64_t)r * ldx + c;
+  *px = accumulate ? *px + v : v;
+}
+
+// per-column max |M[:, c]| of an n x nrhs block as bit patterns (a NaN
+// stays the max): rows strided over the workgroups, one atomic per column
+// per workgroup
+constexpr int kColMaxBlocks = 256;
+__global__ void __launch_bounds__(256) colmax_kernel(int n, int nrhs, const double* __restrict__ M, int ldm,
+                                                     unsigned long long* __restrict__ out) {
+  for (int c0 = 0; c0 < nrhs; c0 += 256) {
+    const int c = c0 + (int)threadIdx.x;
+    if (c < nrhs) {
+      unsigned long long m = 0;
+      for (int r = blockIdx.x; r < n; r += gridDim.x)
+        m = max(m, (unsigned long long)__double_as_longlong(fabs(M[(int64_t)r * ldm + c])));
+      atomicMax(&out[c], m);
+    }
+  }
+}
+
+// max over rows of sum_j |A[i][j]| (the infinity norm), one wavefront per row
+__global__ void __launch_bounds__(256) row_abs_max_kernel(int n, const double* __restrict__ A, int lda,
+                                                          unsigned long long* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned long long m = 0;
+  for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
+    const double* a = A + (int64_t)row * lda;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += fabs(a[c]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    m = max(m, (unsigned long long)__double_as_longlong(s));
+  }
+  if (lane == 0) atomicMax(out, m);
+}
+
+// zero the columns whose flag is set (converged right-hand sides get no correction)
+__global__ void zero_cols_kernel(int n, int nrhs, double* __restrict__ M, int ldm, const int32_t* __restrict__ flag) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * nrhs) return;
+  const int r = (int)(idx / nrhs), c = (int)(idx % nrhs);
+  if (flag[c]) M[(int64_t)r * ldm + c] = 0.0;
+}
+
 // ---- GMRES-IR building blocks (FP64) ------------------------------------------
 // out[i] = V_i . w for the k basis vectors V_i (rows of V, length n, ld n).
 constexpr int kGmDotThreads = 256;
@@ -1069,6 +1121,32 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   } else {
     rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, tmp, 1, x, 1, f.dv,
                                                                                    accumulate);
+  }
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+// D (FP64, n x nrhs, ld ldd) (+)= V U^-1 L^-1 W^T R for a block of
+// right-hand sides: the FP32 solves are the blocked TRSM (MFMA GEMM
+// updates) on the FP32 factor.
+static int mp_solve_mat(nmx_ctx* ctx, MpFactor& f, int nrhs, const double* R, int ldr, double* D, int ldd,
+                        float* T, int accumulate) {
+  const int n = f.n, n4 = n & ~3, h = n4 / 4;
+  const int64_t work = ((int64_t)h + (n - n4)) * nrhs;
+  if (f.pivot) {
+    NMX_TRY((gather_rows<double, float>(ctx, n, nrhs, f.perm, R, ldr, T, nrhs)));
+  } else {
+    rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, R, ldr, T, nrhs, f.dw);
+    NMX_HIP(ctx, hipGetLastError());
+  }
+  NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, n, nrhs, f.F, n, T, nrhs));
+  NMX_TRY(trsm_left<float>(ctx, 1, 0, 0, n, nrhs, f.F, n, T, nrhs));
+  if (f.pivot) {
+    mat_f32_add_f64<<<(unsigned)(((int64_t)n * nrhs + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, T, nrhs, D, ldd,
+                                                                                          accumulate);
+  } else {
+    rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, T, nrhs, D, ldd, f.dv,
+                                                                                  accumulate);
   }
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
@@ -1303,6 +1381,148 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   int dev = nmx_ctx_check(ctx);
   if (dev != NMX_OK) return dev;
   return R->converged ? NMX_OK : NMX_ERR_NOT_CONVERGED;
+}
+
+NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* B, int ldb,
+                               double* X, int ldx, const nmx_ir_cfg* cfg_in, nmx_ir_report* reps) {
+  if (!ctx) return -1;
+  if (n <= 0) return -2;
+  if (nrhs <= 0) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!B) return -6;
+  if (ldb < nrhs) return -7;
+  if (!X) return -8;
+  if (ldx < nrhs) return -9;
+  nmx_ir_cfg cfg;
+  if (cfg_in) {
+    cfg = *cfg_in;
+  } else {
+    nmx_ir_cfg_default(&cfg);
+  }
+  if (cfg.precision < 0 || cfg.precision > 3) return -10;
+  if (cfg.block <= 0) cfg.block = 256;
+  if (cfg.block % 64) return -10;
+  if (n > 65536) return NMX_ERR_UNSUPPORTED;
+  std::vector<nmx_ir_report> local;
+  if (!reps) {
+    local.resize(nrhs);
+    reps = local.data();
+  }
+  for (int c = 0; c < nrhs; c++) reps[c] = nmx_ir_report{};
+
+  MpFactor f;
+  f.pivot = cfg.pivot;
+  int st = mp_alloc(ctx, f, n, cfg.block, cfg.precision);
+  f.pivot = cfg.pivot;
+  void* wsp = nullptr;
+  const size_t blk = (size_t)n * nrhs;
+  if (st == NMX_OK)
+    st = nmx_workspace(ctx, sizeof(double) * blk + sizeof(unsigned long long) * (2 * (size_t)nrhs + 8) +
+                                sizeof(int32_t) * (size_t)nrhs + sizeof(float) * blk + 256,
+                       &wsp);
+  if (st != NMX_OK) {
+    mp_free(ctx, f);
+    return st;
+  }
+  double* R = (double*)wsp;
+  unsigned long long* norms = (unsigned long long*)(R + blk);  // [0, nrhs) |r|, [nrhs, 2 nrhs) |x|, then |A|
+  int32_t* flag = (int32_t*)(norms + 2 * (size_t)nrhs + 8);
+  float* T = (float*)(((uintptr_t)(flag + nrhs) + 255) & ~(uintptr_t)255);
+
+  hipEvent_t e0, e1, e2;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventCreate(&e2);
+  hipEventRecord(e0, ctx->stream);
+  st = mp_factor(ctx, f, A, lda, cfg.seed);
+  hipEventRecord(e1, ctx->stream);
+  if (st == NMX_OK) st = mp_solve_mat(ctx, f, nrhs, B, ldb, X, ldx, T, 0);  // X0 = M^-1 B
+  double anorm = 0.0;
+  if (st == NMX_OK) {
+    zero_u64<<<1, 64, 0, ctx->stream>>>(norms + 2 * nrhs, 1);
+    row_abs_max_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda,
+                                                                                          norms + 2 * nrhs);
+    unsigned long long ha = 0;
+    if (hipMemcpyAsync(&ha, norms + 2 * nrhs, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      st = NMX_ERR_HIP;
+    memcpy(&anorm, &ha, 8);
+  }
+  std::vector<unsigned long long> hn(2 * (size_t)nrhs);
+  std::vector<int32_t> hflag(nrhs, 0);
+  std::vector<double> eta(nrhs, INFINITY), prev(nrhs, INFINITY);
+  std::vector<int> done(nrhs, 0);
+  int it = 0;
+  while (st == NMX_OK) {
+    // R = B - A X in FP64 (MFMA GEMM), then the per-column norms
+    NMX_HIP(ctx, hipMemcpy2DAsync(R, sizeof(double) * nrhs, B, sizeof(double) * ldb, sizeof(double) * nrhs, n,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    {
+      ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n * nrhs, 8.0 * (double)n * n + 24.0 * (double)n * nrhs);
+      st = gemm_launch<double>(ctx, 0, 0, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, nrhs);
+    }
+    if (st != NMX_OK) break;
+    {
+      const int z = 2 * nrhs;
+      for (int o = 0; o < z; o += 64) zero_u64<<<1, 64, 0, ctx->stream>>>(norms + o, std::min(64, z - o));
+    }
+    colmax_kernel<<<std::min(n, kColMaxBlocks), 256, 0, ctx->stream>>>(n, nrhs, R, nrhs, norms);
+    colmax_kernel<<<std::min(n, kColMaxBlocks), 256, 0, ctx->stream>>>(n, nrhs, X, ldx, norms + nrhs);
+    NMX_HIP(ctx, hipGetLastError());
+    NMX_HIP(ctx, hipMemcpyAsync(hn.data(), norms, sizeof(unsigned long long) * hn.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int active = 0;
+    for (int c = 0; c < nrhs; c++) {
+      if (done[c]) continue;
+      double rm, xm;
+      memcpy(&rm, &hn[c], 8);
+      memcpy(&xm, &hn[nrhs + c], 8);
+      eta[c] = rm / (anorm * xm);
+      reps[c].history[it < 32 ? it : 31] = eta[c];
+      reps[c].iterations = it;
+      // converged, out of steps, or stagnating: this column is finished
+      if (!(eta[c] > cfg.tol) || it >= cfg.max_iter || (it > 0 && !(eta[c] < 0.5 * prev[c]))) {
+        done[c] = 1;
+      } else {
+        prev[c] = eta[c];
+        active++;
+      }
+      hflag[c] = done[c];
+    }
+    if (!active) break;
+    NMX_HIP(ctx, hipMemcpyAsync(flag, hflag.data(), sizeof(int32_t) * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    zero_cols_kernel<<<(unsigned)((blk + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, R, nrhs, flag);
+    NMX_HIP(ctx, hipGetLastError());
+    st = mp_solve_mat(ctx, f, nrhs, R, nrhs, X, ldx, T, 1);
+    it++;
+    NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // hflag's host memory is rewritten next step
+  }
+  hipEventRecord(e2, ctx->stream);
+  hipEventSynchronize(e2);
+  float fms = 0.f, rms = 0.f;
+  hipEventElapsedTime(&fms, e0, e1);
+  hipEventElapsedTime(&rms, e1, e2);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipEventDestroy(e2);
+  int32_t hinfo = 0;
+  hipMemcpy(&hinfo, f.info, sizeof(int32_t), hipMemcpyDeviceToHost);
+  int all = 1;
+  for (int c = 0; c < nrhs; c++) {
+    reps[c].backward_error = eta[c];
+    reps[c].converged = eta[c] <= cfg.tol ? 1 : 0;
+    reps[c].factor_ms = fms;
+    reps[c].refine_ms = rms;
+    reps[c].info = hinfo;
+    all &= reps[c].converged;
+  }
+  mp_free(ctx, f);
+  if (st != NMX_OK) return st;
+  int dev = nmx_ctx_check(ctx);
+  if (dev != NMX_OK) return dev;
+  return all ? NMX_OK : NMX_ERR_NOT_CONVERGED;
 }
 
 NMX_API void nmx_ir_cfg_default(nmx_ir_cfg* cfg) {

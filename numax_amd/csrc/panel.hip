@@ -210,14 +210,20 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
       slot_r[par][wave] = wrow;
     }
     __syncthreads();
-    // 3. the workgroup's winner (every wavefront works it out)
-    Key gk = 0;
+    // 3. the workgroup's winner (every wavefront works it out): lane q reads
+    // wavefront q's candidate, then the same key / smallest-row reductions
+    Key ck = 0;
+    int cr = INT_MAX;
+    if (lane < kLeafWaves) {
+      ck = slot_k[par][lane];
+      cr = slot_r[par][lane];
+    }
+    const Key gk0 = PK::wave_max(ck);
+    Key gk = gk0;
     int grow = INT_MAX, gw = 0;
-#pragma unroll
-    for (int q = 0; q < kLeafWaves; q++) {
-      const Key k = slot_k[par][q];
-      const int r = slot_r[par][q];
-      if (k > gk || (k == gk && k != 0 && r < grow)) { gk = k; grow = r; gw = q; }
+    if (gk0 != 0) {  // uniform; rows are distinct, so one lane holds the winner
+      grow = (int)wave_umin(ck == gk0 ? (uint32_t)cr : 0xFFFFFFFFu);
+      gw = __ffsll((long long)__ballot(ck == gk0 && cr == grow)) - 1;
     }
     const T* urow;
     const T* drow;
@@ -261,14 +267,20 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
         stg[dst] = (uint32_t)(wv >> 32);
       }
       __syncthreads();
-      // 6. the global winner: largest key, then smallest row
-      Key bk = 0;
+      // 6. the global winner: largest key, then smallest row (lane q reads
+      // part q's; parts <= 32)
+      Key pk = 0;
+      int pr = INT_MAX;
+      if (lane < parts) {
+        const uint32_t* sq = stg + lane * PSTRIDE;
+        pk = PK::from(sq + W * VW);
+        pr = (int)sq[W * VW + VW];
+      }
+      const Key bk = PK::wave_max(pk);
       int brow = INT_MAX, bp = 0;
-      for (int q = 0; q < parts; q++) {
-        const uint32_t* sq = stg + q * PSTRIDE;
-        const Key k = PK::from(sq + W * VW);
-        const int r = (int)sq[W * VW + VW];
-        if (k > bk || (k == bk && k != 0 && r < brow)) { bk = k; brow = r; bp = q; }
+      if (bk != 0) {
+        brow = (int)wave_umin(pk == bk ? (uint32_t)pr : 0xFFFFFFFFu);
+        bp = __ffsll((long long)__ballot(pk == bk && pr == brow)) - 1;
       }
       gk = bk;
       grow = brow;
@@ -278,6 +290,13 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
 
     const int wr = (gk == 0 || aborted) ? gr : grow;
     if (wr == gr) urow = drow;
+    // FP32: the pivot row into registers once (both rows of the thread use
+    // it); FP64 reads it from LDS per row (registers are at their limit)
+    constexpr bool kRowRegs = sizeof(T) == 4;
+    T u[kRowRegs ? W : 1];
+#pragma unroll
+    for (int c = 0; c < (kRowRegs ? W : 1); c++) u[c] = urow[c];
+    auto urow_at = [&](int c) -> T { return kRowRegs ? u[kRowRegs ? c : 0] : urow[c]; };
     // 7. interchange inside the leaf's columns, record the pivot
 #pragma unroll
     for (int i = 0; i < kLeafRows; i++) {
@@ -285,14 +304,14 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
       if (wr != gr && row < n) {
         if (row == gr) {
 #pragma unroll
-          for (int c = 0; c < W; c++) v[i][c] = urow[c];
+          for (int c = 0; c < W; c++) v[i][c] = urow_at(c);
         } else if (row == wr) {
 #pragma unroll
           for (int c = 0; c < W; c++) v[i][c] = drow[c];
         }
       }
     }
-    T pivot = urow[0];
+    T pivot = urow_at(0);
     if (t == 0 && p == 0) {
       ipiv[gc] = wr;
       if (fabs(pivot) == T(0) && !info_set) {
@@ -301,8 +320,13 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
       }
     }
     if (pivot == T(0)) pivot = T(kPivotFloor);
-    // 8. multipliers and the rank-1 update of the leaf's remaining columns
-    const int rem = w - j;
+    // 8. multipliers and the rank-1 update of the leaf's remaining columns.
+    // The bound is opaque (a vector register), so the guard is a per-element
+    // select rather than 31 uniform branches; columns past it are finished
+    // ones rotated to the end and keep their values (a select, not a
+    // multiply by 0, which an infinite multiplier would turn into NaN).
+    int rem = w - j;
+    asm volatile("" : "+v"(rem));
 #pragma unroll
     for (int i = 0; i < kLeafRows; i++) {
       const int row = row0 + kLeafT * i;
@@ -310,7 +334,10 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
         const T l = v[i][0] / pivot;
         v[i][0] = l;
 #pragma unroll
-        for (int c = 1; c < W; c++) v[i][c] = (c < rem) ? v[i][c] - l * urow[c] : v[i][c];
+        for (int c = 1; c < W; c++) {
+          const T nv = v[i][c] - l * urow_at(c);
+          v[i][c] = c < rem ? nv : v[i][c];
+        }
       }
       rotate_left(v[i]);
     }
@@ -480,12 +507,25 @@ __global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __rest
   __shared__ T Tc[NB][NB + 16 / sizeof(T)];  // Tc[j][i] = op(T)[i][j]
   const int t = threadIdx.x;
   const int v = blockIdx.x * 64 + t;
-  // padded to NB with the identity, so the sweeps below need no bounds
-  for (int idx = t; idx < NB * NB; idx += 64) {
-    const int r = idx / NB, c = idx % NB;
-    const T val = (r < n && c < n) ? Tm[(int64_t)r * ldt + c] : T(r == c ? 1 : 0);  // T[r][c], read along rows
-    if (TRANS) Tc[r][c] = val;  // op(T)[c][r]
-    else Tc[c][r] = val;        // op(T)[r][c]
+  // padded to NB with the identity, so the sweeps below need no bounds; all
+  // NB^2/64 loads of a thread are issued before the first LDS store
+  {
+    constexpr int PER = NB * NB / 64;
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(Tm);
+    T stg[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int idx = t + 64 * i, r = idx / NB, c = idx % NB;  // T[r][c], read along rows
+      const uint32_t off = (uint32_t)r * (uint32_t)ldt + (uint32_t)c;
+      stg[i] = buf_load<T>(rt, (r < n && c < n) ? off * (uint32_t)sizeof(T) : kBufOob);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int idx = t + 64 * i, r = idx / NB, c = idx % NB;
+      const T val = (r < n && c < n) ? stg[i] : T(r == c ? 1 : 0);
+      if (TRANS) Tc[r][c] = val;  // op(T)[c][r]
+      else Tc[c][r] = val;        // op(T)[r][c]
+    }
   }
   __syncthreads();
   // the block's 64 vectors through one descriptor (32-bit offsets, past-the-

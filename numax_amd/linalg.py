@@ -353,6 +353,12 @@ def gemm(a, b, c=None, alpha: float = 1.0, beta: float = 0.0, transa: bool = Fal
 SOLVERS = {"auto": 0, "classical": 1, "gmres": 2}
 
 
+def _report(rep) -> dict:
+    return {"iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations, "converged": bool(rep.converged),
+            "backward_error": rep.backward_error, "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms,
+            "info": rep.info, "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
+
+
 def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block: int = 256,
                seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial", solver: str = "auto",
                gmres_restart: int = 30):
@@ -395,15 +401,24 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
     cfg.solver = SOLVERS[solver]
     cfg.gmres_restart = gmres_restart
     ctx = _ctx(A)
+    if B.dim() == 2 and B.shape[1] > 1:
+        # one factorization, all columns refined together
+        if cfg.solver == SOLVERS["gmres"]:
+            raise NumaxError("solve_mpir: GMRES-IR takes one right-hand side at a time")
+        nrhs = int(B.shape[1])
+        X = torch.empty_like(B)
+        reps = (_lib.IrReport * nrhs)()
+        st = lib.nmx_gesv_mpir_nrhs(ctx.handle, n, nrhs, _ptr(A), n, _ptr(B), nrhs, _ptr(X), nrhs,
+                                    ctypes.byref(cfg), reps)
+        if st != 0 and not (st == 5 and not raise_on_failure):
+            _lib.check(st, "solve_mpir")
+        return X, [_report(r) for r in reps]
     xs, reports = [], []
     for col in cols:
         x = torch.empty_like(col)
         rep = _lib.IrReport()
         st = lib.nmx_gesv_mpir(ctx.handle, n, _ptr(A), n, _ptr(col), _ptr(x), ctypes.byref(cfg), ctypes.byref(rep))
-        info = {"iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations,
-                "converged": bool(rep.converged), "backward_error": rep.backward_error,
-                "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms, "info": rep.info,
-                "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
+        info = _report(rep)
         if st != 0 and not (st == 5 and not raise_on_failure):
             _lib.check(st, "solve_mpir")
         xs.append(x)

@@ -138,3 +138,22 @@ def test_gmres_ir_odd_sizes(n):
     x, rep = nl.solve_mpir(a, b, precision="fp16", solver="gmres", gmres_restart=8, max_iter=30)
     eta = oracle.backward_error(a, x.cpu().numpy(), b)
     assert rep["converged"] and eta <= 1e-15, (rep, eta)
+
+
+@pytest.mark.parametrize("nrhs,kappa,pivot", [(3, 1e4, "partial"), (64, 1e6, "partial"), (5, 1e2, "butterfly")])
+def test_mpir_block_of_right_hand_sides(nrhs, kappa, pivot):
+    """One factorization, every column refined to FP64 backward error; each
+    column agrees with its single-right-hand-side solve to what kappa allows."""
+    n = 1001 if pivot == "partial" else 512
+    a = svd_matrix(n, kappa, seed=5)
+    b = R.normal(6, (n, nrhs))
+    x, reps = nl.solve_mpir(a, b, precision="fp16s", pivot=pivot)
+    x = x.cpu().numpy()
+    assert len(reps) == nrhs
+    for j in (0, nrhs // 2, nrhs - 1):
+        eta = oracle.backward_error(a, x[:, j], b[:, j])
+        assert reps[j]["converged"] and eta <= 1e-15, (j, reps[j], eta)
+        xj, _ = nl.solve_mpir(a, b[:, j], precision="fp16s", pivot=pivot)
+        xj = xj.cpu().numpy()
+        assert np.max(np.abs(x[:, j] - xj)) / np.max(np.abs(xj)) < 10 * kappa * 1e-15
+    assert max(r["backward_error"] for r in reps) <= 1e-15
