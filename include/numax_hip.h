@@ -229,6 +229,24 @@ int nmx_qr_r_f64(nmx_ctx* ctx, int n, const double* QR, int lda, double* R, int 
  * place -- X is B's first n rows. */
 int nmx_gels_f64(nmx_ctx* ctx, int m, int n, int nrhs, double* A, int lda, double* tau, double* B, int ldb, int block);
 
+/* ---- graph capture ----------------------------------------------------------
+ * Records the library calls made on the context's stream between
+ * nmx_graph_begin and nmx_graph_end into one HIP graph; nmx_graph_launch
+ * replays them on that stream with one launch.  Only calls that neither
+ * synchronize nor allocate nor advance host-side state can be captured:
+ * the batched solvers (whose refinement loops run inside their kernels),
+ * the batched eigensolver and least squares, QR, GEMM, TRSM.  A call that
+ * would need more workspace than an earlier call left (run it once first),
+ * the pivoting LU drivers (their cooperative panels take host-side epochs)
+ * and the mixed-precision drivers (their loops read norms on the host)
+ * return NMX_ERR_UNSUPPORTED while capturing.  No reference counterpart:
+ * for callers that replay the same batch shape (serving loops). */
+typedef struct nmx_graph nmx_graph;
+int nmx_graph_begin(nmx_ctx* ctx);
+int nmx_graph_end(nmx_ctx* ctx, nmx_graph** out);
+int nmx_graph_launch(nmx_graph* graph);
+int nmx_graph_destroy(nmx_graph* graph);
+
 /* ---- synthetic inputs ------------------------------------------------------
  * Philox4x32-10 stream (the same words as numax_amd.random): kind 0 fills
  * lo + (hi - lo) U[0,1), kind 1 fills N(0,1) by Box--Muller.  Element e of

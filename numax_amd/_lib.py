@@ -49,6 +49,10 @@ SIGNATURES = {
     "nmx_potrs_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_gesv_mpir_nrhs": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int, _c_vp,
                                     _c_vp]),
+    "nmx_graph_begin": (_c_int, [_c_vp]),
+    "nmx_graph_end": (_c_int, [_c_vp, ctypes.POINTER(_c_vp)]),
+    "nmx_graph_launch": (_c_int, [_c_vp]),
+    "nmx_graph_destroy": (_c_int, [_c_vp]),
     "nmx_geqrf_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_ormqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_orgqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),

@@ -90,12 +90,14 @@ NMX_API int nmx_ctx_set_stream(nmx_ctx* c, void* stream) {
 
 NMX_API int nmx_sync(nmx_ctx* c) {
   if (!c) return -1;
+  if (c->capturing) return NMX_ERR_UNSUPPORTED;
   NMX_HIP(c, hipStreamSynchronize(c->stream));
   return NMX_OK;
 }
 
 NMX_API int nmx_ctx_check(nmx_ctx* c) {
   if (!c) return -1;
+  if (c->capturing) return NMX_ERR_UNSUPPORTED;
   int h = 0;
   NMX_HIP(c, hipMemcpyAsync(&h, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   NMX_HIP(c, hipStreamSynchronize(c->stream));
@@ -111,6 +113,7 @@ NMX_API int nmx_ctx_check(nmx_ctx* c) {
 int nmx_workspace(nmx_ctx* c, size_t bytes, void** out) {
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes > c->ws_bytes) {
+    if (c->capturing) return NMX_ERR_UNSUPPORTED;  // run the call once before capturing it
     NMX_HIP(c, hipStreamSynchronize(c->stream));
     if (c->ws) hipFree(c->ws);
     c->ws = nullptr;
@@ -128,6 +131,8 @@ int nmx_mailbox(nmx_ctx* c, void** out) {
 }
 
 int nmx_epoch_reserve(nmx_ctx* c, uint32_t columns, uint32_t* base) {
+  // epochs are kernel arguments: a replayed graph would reuse them
+  if (c->capturing) return NMX_ERR_UNSUPPORTED;
   if ((uint64_t)c->epoch + columns + 4 >= 0x7fffffffu) {
     size_t words = kMailboxBytes / 4;
     fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
@@ -268,4 +273,53 @@ NMX_API const char* nmx_kernel_name(int kind) {
                                        "mp_gemm",    "mp_panel",   "residual", "mp_trsv",   "other",
                                        "batched_solve", "batched_eigh", "batched_lstsq", "qr_panel"};
   return (kind >= 0 && kind < K_COUNT) ? names[kind] : "unknown";
+}
+
+// ---- graph capture of the context stream -------------------------------------
+struct nmx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+NMX_API int nmx_graph_begin(nmx_ctx* c) {
+  if (!c) return -1;
+  if (c->capturing || c->profiling) return NMX_ERR_UNSUPPORTED;
+  NMX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  return NMX_OK;
+}
+
+NMX_API int nmx_graph_end(nmx_ctx* c, nmx_graph** out) {
+  if (!c) return -1;
+  if (!out) return -2;
+  *out = nullptr;
+  if (!c->capturing) return NMX_ERR_UNSUPPORTED;
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  NMX_HIP(c, hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t e = nullptr;
+  if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return NMX_ERR_HIP;
+  }
+  nmx_graph* r = new nmx_graph;
+  r->graph = g;
+  r->exec = e;
+  r->stream = c->stream;
+  *out = r;
+  return NMX_OK;
+}
+
+NMX_API int nmx_graph_launch(nmx_graph* g) {
+  if (!g) return -1;
+  return hipGraphLaunch(g->exec, g->stream) == hipSuccess ? NMX_OK : NMX_ERR_HIP;
+}
+
+NMX_API int nmx_graph_destroy(nmx_graph* g) {
+  if (!g) return NMX_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return NMX_OK;
 }

@@ -1267,6 +1267,7 @@ static int gmres_correction(nmx_ctx* ctx, MpFactor& f, const double* A, int lda,
 NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x,
                           const nmx_ir_cfg* cfg_in, nmx_ir_report* rep) {
   if (!ctx) return -1;
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;  // the refinement loop reads norms on the host
   if (n <= 0) return -2;
   if (!A) return -3;
   if (lda < n) return -4;
@@ -1386,6 +1387,7 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
 NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* B, int ldb,
                                double* X, int ldx, const nmx_ir_cfg* cfg_in, nmx_ir_report* reps) {
   if (!ctx) return -1;
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;
   if (n <= 0) return -2;
   if (nrhs <= 0) return -3;
   if (!A) return -4;

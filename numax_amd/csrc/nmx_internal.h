@@ -78,6 +78,9 @@ struct nmx_ctx {
   // buffers of the mixed-precision factorization, kept across calls
   void* mp_cache = nullptr;
   void (*mp_cache_free)(void*) = nullptr;
+  // the stream is being captured into a graph (nmx_graph_begin .. _end):
+  // nothing may synchronize, allocate or advance host-side epochs
+  bool capturing = false;
   // event profiler (off unless nmx_ctx_set_profiling)
   bool profiling = false;
   std::vector<ProfRec> recs;

@@ -494,6 +494,69 @@ def batched_solve(a, b, n, kind, offsets_a, offsets_b, tol: float = 1e-15, max_i
     return BatchResult(X, info, iters, berr, np.asarray(offsets_b), np.asarray(n))
 
 
+class BatchedSolveGraph:
+    """One ``nmx_batched_gesv`` captured as a HIP graph for a fixed batch shape
+    (counts, sizes, offsets): ``replay()`` re-solves whatever values ``a`` and
+    ``b`` hold at that moment into ``result`` with one launch.  Built by
+    :func:`batched_solve_graph`; the device buffers stay owned here."""
+
+    def __init__(self, handle, keep, result: BatchResult, a: torch.Tensor, b: torch.Tensor):
+        self._handle = handle
+        self._keep = keep
+        self.result = result
+        self.a = a
+        self.b = b
+
+    def replay(self) -> BatchResult:
+        _lib.check(_lib.load().nmx_graph_launch(self._handle), "graph replay")
+        return self.result
+
+    def close(self) -> None:
+        if self._handle:
+            _lib.load().nmx_graph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batched_solve_graph(a, b, n, kind, offsets_a, offsets_b, tol: float = 1e-15, max_iter: int = 10,
+                        teams_per_cu=None) -> BatchedSolveGraph:
+    """:func:`batched_solve` run once eagerly (which also sizes the workspace),
+    then captured into a graph on the same stream for replays; the eager
+    result is in ``graph.result`` until the first replay overwrites it."""
+    A = _device_tensor(a, torch.float64).clone()
+    B = _device_tensor(b, torch.float64).clone()
+    res = batched_solve(A, B, n, kind, offsets_a, offsets_b, tol=tol, max_iter=max_iter, teams_per_cu=teams_per_cu)
+    count = len(n)
+    N = _device_tensor(np.asarray(n, dtype=np.int32))
+    K = _device_tensor(np.asarray(kind, dtype=np.uint8))
+    OA = _device_tensor(np.asarray(offsets_a, dtype=np.int64))
+    OB = _device_tensor(np.asarray(offsets_b, dtype=np.int64))
+    lib = _lib.load()
+    cfg = _lib.BatchCfg()
+    lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+    cfg.tol = float(tol)
+    cfg.max_iter = int(max_iter)
+    if teams_per_cu is not None:
+        for i, v in enumerate(teams_per_cu):
+            cfg.teams_per_cu[i] = int(v)
+    ctx = _ctx(A)
+    torch.cuda.synchronize(A.device)
+    _lib.check(lib.nmx_graph_begin(ctx.handle), "graph begin")
+    st = lib.nmx_batched_gesv(ctx.handle, count, _ptr(N), _ptr(OA), _ptr(OB), _ptr(K), _ptr(A), A.numel(), _ptr(B),
+                              B.numel(), _ptr(res.x), _ptr(res.info), _ptr(res.iters), _ptr(res.berr),
+                              ctypes.byref(cfg))
+    handle = ctypes.c_void_p()
+    st_end = lib.nmx_graph_end(ctx.handle, ctypes.byref(handle))
+    _lib.check(st, "batched_solve (captured)")
+    _lib.check(st_end, "graph end")
+    return BatchedSolveGraph(handle, (N, K, OA, OB, cfg), res, A, B)
+
+
 # ---- batched small-matrix eigensolver and tall-skinny least squares --------------
 def eigh_batched(a, n, offsets_a, offsets_w, sweeps: int = 12, vectors: bool = True):
     """numax's register-tier ``eigh`` (numax/linalg/_array/eigen.mojo:316-392)
