@@ -141,6 +141,10 @@ typedef struct nmx_ir_report {
 void nmx_ir_cfg_default(nmx_ir_cfg* cfg);
 int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x, const nmx_ir_cfg* cfg,
                   nmx_ir_report* report);
+/* Autotuner of the panel width (no reference counterpart): times the
+ * factorization of a synthetic N(0,1) matrix of order n at block 128, 256
+ * and 512 and writes the fastest into cfg->block (other fields as given). */
+int nmx_ir_autotune(nmx_ctx* ctx, int n, nmx_ir_cfg* cfg);
 /* The same solve for an n x nrhs block of right-hand sides (row-major B, X
  * with leading dimensions ldb, ldx >= nrhs): one factorization, then every
  * column refined together -- the FP64 residual R = B - A X is one MFMA GEMM
@@ -184,6 +188,13 @@ void nmx_batch_cfg_default(nmx_batch_cfg* cfg);
 int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
                      const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems, double* X,
                      int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg);
+
+/* Autotuner of the launch shape (no reference counterpart): for each class
+ * (n > 128, 64 < n <= 128, n <= 64) times a synthetic batch of `samples`
+ * problems of that class at 1, 2, 3, 4, 6 and 8 resident workgroups per CU
+ * and writes the fastest into cfg->teams_per_cu; cfg's other fields are
+ * used as given.  Allocates its own scratch; a few seconds at samples ~ 2000. */
+int nmx_batch_autotune(nmx_ctx* ctx, int samples, nmx_batch_cfg* cfg);
 
 /* ---- batched small symmetric eigensolver -----------------------------------
  * The register-tier eigh (numax/linalg/_array/eigen.mojo:316-392) that

@@ -53,6 +53,8 @@ SIGNATURES = {
     "nmx_graph_end": (_c_int, [_c_vp, ctypes.POINTER(_c_vp)]),
     "nmx_graph_launch": (_c_int, [_c_vp]),
     "nmx_graph_destroy": (_c_int, [_c_vp]),
+    "nmx_batch_autotune": (_c_int, [_c_vp, _c_int, _c_vp]),
+    "nmx_ir_autotune": (_c_int, [_c_vp, _c_int, _c_vp]),
     "nmx_geqrf_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_ormqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_orgqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
@@ -110,6 +112,10 @@ def load() -> ctypes.CDLL:
                 raise RuntimeError(
                     f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
                     "(numax_amd has no CPU fallback)")
+            # One HIP runtime per process: PyTorch ships its own libamdhip64, so
+            # it is loaded first and the library binds to it (same soname);
+            # loaded the other way round the two runtimes see no common device.
+            import torch  # noqa: F401
             lib = ctypes.CDLL(LIB_PATH)
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(lib, name)

@@ -30,6 +30,7 @@
 #include <math.h>
 
 #include <type_traits>
+#include <vector>
 
 #include "nmx_internal.h"
 
@@ -1188,4 +1189,115 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
     NMX_TRY((launch_class<1, true>(ctx, a, cfg.teams_per_cu[2])));
   }
   return NMX_OK;
+}
+
+// ---- autotuner: resident workgroups per CU, per size class ---------------------
+// For each of the three launch classes (n > 128, 64 < n <= 128, n <= 64) a
+// synthetic batch of `samples` problems of that class (n uniform in the
+// class, half LU with a diagonal shift, half Cholesky on a diagonally
+// dominant symmetric matrix) is solved once per candidate value and timed
+// with events; the fastest value goes into cfg->teams_per_cu.  The other
+// fields of *cfg are kept (tol, max_iter shape the refinement being timed).
+NMX_API int nmx_batch_autotune(nmx_ctx* ctx, int samples, nmx_batch_cfg* cfg) {
+  if (!ctx) return -1;
+  if (samples < 1 || samples > 1000000) return -2;
+  if (!cfg) return -3;
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;
+  static const int kCand[] = {1, 2, 3, 4, 6, 8};
+  const int lo[3] = {129, 65, 1}, hi[3] = {256, 128, 64};
+  uint64_t state = 0x9E3779B97F4A7C15ull;
+  auto next = [&state]() {  // splitmix64
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  auto unif = [&next]() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0; };
+  int status = NMX_OK;
+  for (int cls = 0; cls < 3 && status == NMX_OK; cls++) {
+    std::vector<int32_t> n(samples);
+    std::vector<uint8_t> kind(samples);
+    std::vector<int64_t> oa(samples), ob(samples);
+    int64_t ae = 0, be = 0;
+    for (int p = 0; p < samples; p++) {
+      n[p] = lo[cls] + (int)(next() % (uint64_t)(hi[cls] - lo[cls] + 1));
+      kind[p] = (uint8_t)(p & 1);
+      oa[p] = ae;
+      ob[p] = be;
+      ae += (int64_t)n[p] * n[p];
+      be += n[p];
+    }
+    std::vector<double> a(ae), b(be);
+    for (int p = 0; p < samples; p++) {
+      const int m = n[p];
+      double* ap = a.data() + oa[p];
+      for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) ap[(int64_t)i * m + j] = unif();
+      if (kind[p])
+        for (int i = 0; i < m; i++)
+          for (int j = 0; j < i; j++) ap[(int64_t)j * m + i] = ap[(int64_t)i * m + j];
+      for (int i = 0; i < m; i++) ap[(int64_t)i * m + i] += kind[p] ? (double)m : 0.1 * m;
+      for (int i = 0; i < m; i++) b[ob[p] + i] = unif();
+    }
+    void* mem = nullptr;
+    const size_t bytes = 4 * (size_t)samples * 3 + 8 * (size_t)samples * 3 + (size_t)samples + 8 * (size_t)(ae + 2 * be) +
+                         1024;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return NMX_ERR_ALLOC;
+    char* q = (char*)mem;
+    auto carve_b = [&q](size_t sz) {
+      char* r = q;
+      q += (sz + 255) & ~(size_t)255;
+      return (void*)r;
+    };
+    int32_t* dn = (int32_t*)carve_b(4 * (size_t)samples);
+    int32_t* dinfo = (int32_t*)carve_b(4 * (size_t)samples);
+    int32_t* diters = (int32_t*)carve_b(4 * (size_t)samples);
+    int64_t* doa = (int64_t*)carve_b(8 * (size_t)samples);
+    int64_t* dob = (int64_t*)carve_b(8 * (size_t)samples);
+    double* dberr = (double*)carve_b(8 * (size_t)samples);
+    uint8_t* dk = (uint8_t*)carve_b((size_t)samples);
+    double* dA = (double*)carve_b(8 * (size_t)ae);
+    double* dB = (double*)carve_b(8 * (size_t)be);
+    double* dX = (double*)carve_b(8 * (size_t)be);
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMemcpyAsync(dn, n.data(), 4 * (size_t)samples, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(doa, oa.data(), 8 * (size_t)samples, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dob, ob.data(), 8 * (size_t)samples, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dk, kind.data(), (size_t)samples, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dA, a.data(), 8 * (size_t)ae, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dB, b.data(), 8 * (size_t)be, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      status = NMX_ERR_HIP;
+    float best_ms = INFINITY;
+    int best = cfg->teams_per_cu[cls];
+    for (int ci = 0; ci < (int)(sizeof(kCand) / sizeof(kCand[0])) && status == NMX_OK; ci++) {
+      nmx_batch_cfg c = *cfg;
+      c.teams_per_cu[cls] = kCand[ci];
+      float tot = 0.f;
+      for (int rep = 0; rep < 3 && status == NMX_OK; rep++) {  // the first run is a warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) status = NMX_ERR_HIP;
+        if (status == NMX_OK)
+          status = nmx_batched_gesv(ctx, samples, dn, doa, dob, dk, dA, ae, dB, be, dX, dinfo, diters, dberr, &c);
+        if (status == NMX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+          status = NMX_ERR_HIP;
+        float ms = 0.f;
+        if (status == NMX_OK && rep > 0) {
+          hipEventElapsedTime(&ms, e0, e1);
+          tot += ms;
+        }
+      }
+      if (status == NMX_OK && tot < best_ms) {
+        best_ms = tot;
+        best = kCand[ci];
+      }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipStreamSynchronize(s);
+    hipFree(mem);
+    if (status == NMX_OK) cfg->teams_per_cu[cls] = best;
+  }
+  if (status != NMX_OK) return status;
+  return nmx_ctx_check(ctx);
 }

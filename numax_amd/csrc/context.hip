@@ -1,4 +1,5 @@
 // Context, workspace, mailbox, status strings and the Philox input generator.
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 
@@ -28,28 +29,45 @@ NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
   if (!out) return -3;
   *out = nullptr;
   int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return NMX_ERR_HIP;
+  const hipError_t ec = hipGetDeviceCount(&count);
+  if (ec != hipSuccess || count <= 0) {
+    fprintf(stderr, "nmx_ctx_create: no HIP device (%s)\n", hipGetErrorString(ec));
+    return NMX_ERR_HIP;
+  }
   if (device < 0 || device >= count) return -1;
   nmx_ctx* c = new nmx_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess) { delete c; return NMX_ERR_HIP; }
+  if (hipSetDevice(device) != hipSuccess) {
+    fprintf(stderr, "nmx_ctx_create: hipSetDevice(%d) failed\n", device);
+    delete c;
+    return NMX_ERR_HIP;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return NMX_ERR_HIP; }
   c->num_cus = prop.multiProcessorCount;
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
-    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return NMX_ERR_HIP; }
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+      fprintf(stderr, "nmx_ctx_create: hipStreamCreate failed\n");
+      delete c;
+      return NMX_ERR_HIP;
+    }
     c->own_stream = true;
   }
   if (hipMalloc(&c->err, 256) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
   if (hipMalloc(&c->mailbox, kMailboxBytes) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
   if (hipMalloc(&c->tri_inv, 64 * 64 * sizeof(double)) != hipSuccess) { nmx_ctx_destroy(c); return NMX_ERR_ALLOC; }
   c->mailbox_bytes = kMailboxBytes;
+  // an error another runtime user of this process left behind is not ours
+  (void)hipGetLastError();
   size_t words = kMailboxBytes / 4;
   fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
   fill_u32<<<1, 64, 0, c->stream>>>((uint32_t*)c->err, 64, 0u);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    fprintf(stderr, "nmx_ctx_create: initializing the context on device %d: %s\n", device, hipGetErrorString(e));
     nmx_ctx_destroy(c);
     return NMX_ERR_HIP;
   }

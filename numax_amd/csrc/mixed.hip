@@ -1527,6 +1527,53 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   return all ? NMX_OK : NMX_ERR_NOT_CONVERGED;
 }
 
+// Autotuner of the panel width: one factorization (plus the first solve and
+// residual) of a synthetic N(0,1) matrix of order n per candidate width, the
+// fastest factorization time's width into cfg->block.  Other fields kept.
+NMX_API int nmx_ir_autotune(nmx_ctx* ctx, int n, nmx_ir_cfg* cfg) {
+  if (!ctx) return -1;
+  if (n < 64 || n > 65536) return -2;
+  if (!cfg) return -3;
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;
+  static const int kCand[] = {128, 256, 512};
+  double *A = nullptr, *b = nullptr, *x = nullptr;
+  if (hipMalloc(&A, sizeof(double) * (size_t)n * n) != hipSuccess) return NMX_ERR_ALLOC;
+  if (hipMalloc(&b, sizeof(double) * (size_t)n) != hipSuccess || hipMalloc(&x, sizeof(double) * (size_t)n) != hipSuccess) {
+    hipFree(A);
+    if (b) hipFree(b);
+    return NMX_ERR_ALLOC;
+  }
+  int st = nmx_fill_random_f64(ctx, 1, 0x7ab1eULL, 0, (int64_t)n * n, 0.0, 0.0, A);
+  if (st == NMX_OK) st = nmx_fill_random_f64(ctx, 1, 0x7ab1fULL, 0, n, 0.0, 0.0, b);
+  double best = INFINITY;
+  int best_block = cfg->block > 0 ? cfg->block : 256;
+  for (int ci = 0; ci < 3 && st == NMX_OK; ci++) {
+    if (kCand[ci] > n) continue;
+    nmx_ir_cfg c = *cfg;
+    c.block = kCand[ci];
+    c.max_iter = 0;
+    c.solver = 1;
+    double tot = 0.0;
+    for (int rep = 0; rep < 3 && st == NMX_OK; rep++) {  // the first run is a warm-up
+      nmx_ir_report r;
+      const int s2 = nmx_gesv_mpir(ctx, n, A, n, b, x, &c, &r);
+      if (s2 != NMX_OK && s2 != NMX_ERR_NOT_CONVERGED) st = s2;
+      if (rep > 0) tot += r.factor_ms;
+    }
+    if (st == NMX_OK && tot < best) {
+      best = tot;
+      best_block = kCand[ci];
+    }
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(A);
+  hipFree(b);
+  hipFree(x);
+  if (st != NMX_OK) return st;
+  cfg->block = best_block;
+  return NMX_OK;
+}
+
 NMX_API void nmx_ir_cfg_default(nmx_ir_cfg* cfg) {
   if (!cfg) return;
   cfg->precision = 2;

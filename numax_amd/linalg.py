@@ -359,14 +359,16 @@ def _report(rep) -> dict:
             "info": rep.info, "history": [rep.history[i] for i in range(min(rep.iterations + 1, 32))]}
 
 
-def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block: int = 256,
+def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block=None,
                seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial", solver: str = "auto",
                gmres_restart: int = 30):
     """``x`` with ``A x = b`` to FP64 backward error from a low-precision
     factorization (no reference counterpart; numax's ``solve`` is the FP64
     path).  ``precision`` picks the trailing-update arithmetic: ``"fp16"``,
     ``"bf16"``, ``"fp16s"`` (FP16 hi + lo split, ~FP32 accuracy) or ``"fp32"``.
-    ``solver``: ``"classical"`` refinement, ``"gmres"`` (GMRES-IR: FP64 GMRES
+    ``block``: the panel width (a multiple of 64; None: the autotuned width
+    cached for this order and precision, else 256).  ``solver``:
+    ``"classical"`` refinement, ``"gmres"`` (GMRES-IR: FP64 GMRES
     on the correction equation, preconditioned by the low-precision factor)
     or ``"auto"`` (classical while each step at least halves the backward
     error, GMRES-IR from the first step that does not).  Returns ``(x, report)``; ``report`` holds the
@@ -391,7 +393,10 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
     cfg.precision = PRECISIONS[precision]
     cfg.tol = tol
     cfg.max_iter = max_iter
-    cfg.block = block
+    if block is None:
+        from numax_amd import autotune
+        block = autotune.cached("mpir", A.device.index or 0, n=n, precision=precision) or 256
+    cfg.block = int(block)
     cfg.seed = seed
     if pivot not in ("partial", "butterfly"):
         raise NumaxError("solve_mpir: pivot must be 'partial' or 'butterfly'")
@@ -485,6 +490,9 @@ def batched_solve(a, b, n, kind, offsets_a, offsets_b, tol: float = 1e-15, max_i
     lib.nmx_batch_cfg_default(ctypes.byref(cfg))
     cfg.tol = float(tol)
     cfg.max_iter = int(max_iter)
+    if teams_per_cu is None:
+        from numax_amd import autotune
+        teams_per_cu = autotune.cached("batched", A.device.index or 0)
     if teams_per_cu is not None:
         for i, v in enumerate(teams_per_cu):
             cfg.teams_per_cu[i] = int(v)

@@ -157,3 +157,19 @@ def test_mpir_block_of_right_hand_sides(nrhs, kappa, pivot):
         xj = xj.cpu().numpy()
         assert np.max(np.abs(x[:, j] - xj)) / np.max(np.abs(xj)) < 10 * kappa * 1e-15
     assert max(r["backward_error"] for r in reps) <= 1e-15
+
+
+def test_autotuners_pick_from_their_candidates(tmp_path, monkeypatch):
+    from numax_amd import autotune
+    monkeypatch.setenv("NUMAX_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    block = autotune.tune_mpir(2048)
+    assert block in (128, 256, 512)
+    assert autotune.cached("mpir", 0, n=2048, precision="fp16s") == block
+    teams = autotune.tune_batched(samples=300)
+    assert len(teams) == 3 and all(t in (1, 2, 3, 4, 6, 8) for t in teams)
+    assert autotune.cached("batched", 0) == teams
+    # the tuned shapes are the ones the solvers now use, and still solve
+    a = svd_matrix(2048, 1e4, seed=1)
+    b = R.normal(2, (2048,))
+    x, rep = nl.solve_mpir(a, b)
+    assert rep["converged"] and oracle.backward_error(a, x.cpu().numpy(), b) <= 1e-15
