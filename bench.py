@@ -173,6 +173,13 @@ def batched_leg(args, torch) -> dict:
     berr = torch.zeros(len(n), dtype=torch.float64, device="cuda")
     cfg = _lib.BatchCfg()
     lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+    teams = None
+    if not args.no_autotune:
+        from numax_amd import autotune
+        teams = autotune.tune_batched(samples=1500, store=False)
+        for i, v in enumerate(teams):
+            cfg.teams_per_cu[i] = v
+        log(f"autotuned teams_per_cu {teams}")
     P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
 
     def step():
@@ -204,6 +211,7 @@ def batched_leg(args, torch) -> dict:
                         "unit": "GB/s", "frac": bytes_alg / kern_s / 1e9 / PEAK["hbm_GBs"],
                         "traffic": MEASURED_TRAFFIC.get(("batched_solve", len(n))),
                         "bytes_per_step": bytes_alg, "kernel_ms": rec["ms"]},
+           "teams_per_cu": [int(cfg.teams_per_cu[i]) for i in range(3)], "autotuned": teams is not None,
            "max_backward_error": float(np.nanmax(be)), "max_iterations": int(it.max()),
            "failed": int((inf != 0).sum()), "kernel_profile": prof}
     log("batched: " + json.dumps({k: v for k, v in out.items() if k != "kernel_profile"}))
@@ -221,6 +229,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-15)
     ap.add_argument("--all-kappa", action="store_true", help="also time kappa = 1e2, 1e4, 1e6")
     ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the library's default launch shapes instead of tuning them first")
     ap.add_argument("--block-rhs", type=int, default=64,
                     help="also time one factor + refine of this many right-hand sides against plain FP64 (0: skip)")
     ap.add_argument("--block", type=int, default=256)
@@ -295,6 +305,11 @@ def main() -> int:
     lib.nmx_ir_cfg_default(ctypes.byref(cfg))
     cfg.precision = nl.PRECISIONS[args.precision]
     cfg.block = args.block
+    if not args.no_autotune:
+        lib_st = lib.nmx_ir_autotune(ctx.handle, n, ctypes.byref(cfg))
+        _lib.check(lib_st, "nmx_ir_autotune")
+        log(f"autotuned mixed-precision panel width {cfg.block}")
+    mp_block = int(cfg.block)
     cfg.tol = args.tol
     rep = _lib.IrReport()
     xm = torch.empty(n, dtype=torch.float64, device=dev)
@@ -405,8 +420,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"mixed-precision LU-IR solve, n={n}, 1 RHS, A = U diag(sigma) V^T (Haar U, V), "
                                f"kappa={args.kappa:g}, partial pivoting, {args.precision} trailing update, "
-                               f"block {args.block}, refined to FP64 backward error <= {args.tol:g}",
-                   "n": n, "nrhs": 1, "kappa": args.kappa, "precision": args.precision, "block": args.block},
+                               f"block {mp_block}, refined to FP64 backward error <= {args.tol:g}",
+                   "n": n, "nrhs": 1, "kappa": args.kappa, "precision": args.precision, "block": mp_block,
+                   "autotuned": not args.no_autotune},
         "tflops": head["effective_tflops"],
         "backward_error": head["backward_error"],
         "ir_iterations": head["iterations"],
