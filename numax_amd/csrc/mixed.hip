@@ -1126,11 +1126,35 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   return NMX_OK;
 }
 
+// Y := op^-1 T for nrhs right-hand sides (n x nrhs, ld nrhs; T is used as
+// scratch), op = L (unit lower) or U, right-looking over the 128-row blocks
+// of the factorization: each diagonal block is solved by its precomputed
+// inverse (one GEMM), then the blocks still to solve are updated with one
+// GEMM -- two MFMA launches per block, none of them a long dependent chain.
+static int mp_trsm_blocks(nmx_ctx* ctx, MpFactor& f, int upper, int nrhs, float* T, float* Y) {
+  const int n = f.n, nblk = ceil_div(n, TB);
+  const float* inv = upper ? f.invU : f.invL;
+  for (int s = 0; s < nblk; s++) {
+    const int kb = upper ? nblk - 1 - s : s;
+    const int r0 = kb * TB, bs = std::min(TB, n - r0);
+    NMX_TRY(gemm_launch<float>(ctx, 0, 0, bs, nrhs, bs, 1.0f, inv + (int64_t)kb * TB * TB, TB,
+                               T + (int64_t)r0 * nrhs, nrhs, 0.0f, Y + (int64_t)r0 * nrhs, nrhs));
+    if (!upper && r0 + bs < n)
+      NMX_TRY(gemm_launch<float>(ctx, 0, 0, n - r0 - bs, nrhs, bs, -1.0f, f.F + (int64_t)(r0 + bs) * n + r0, n,
+                                 Y + (int64_t)r0 * nrhs, nrhs, 1.0f, T + (int64_t)(r0 + bs) * nrhs, nrhs));
+    if (upper && r0 > 0)
+      NMX_TRY(gemm_launch<float>(ctx, 0, 0, r0, nrhs, bs, -1.0f, f.F + r0, n, Y + (int64_t)r0 * nrhs, nrhs, 1.0f, T,
+                                 nrhs));
+  }
+  return NMX_OK;
+}
+
 // D (FP64, n x nrhs, ld ldd) (+)= V U^-1 L^-1 W^T R for a block of
-// right-hand sides: the FP32 solves are the blocked TRSM (MFMA GEMM
-// updates) on the FP32 factor.
+// right-hand sides: the FP32 solves are blocked over the factorization's
+// 128 x 128 diagonal-block inverses with MFMA GEMM updates.  T and Y are
+// n x nrhs FP32 scratch.
 static int mp_solve_mat(nmx_ctx* ctx, MpFactor& f, int nrhs, const double* R, int ldr, double* D, int ldd,
-                        float* T, int accumulate) {
+                        float* T, float* Y, int accumulate) {
   const int n = f.n, n4 = n & ~3, h = n4 / 4;
   const int64_t work = ((int64_t)h + (n - n4)) * nrhs;
   if (f.pivot) {
@@ -1139,8 +1163,8 @@ static int mp_solve_mat(nmx_ctx* ctx, MpFactor& f, int nrhs, const double* R, in
     rbt_vec_t_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, R, ldr, T, nrhs, f.dw);
     NMX_HIP(ctx, hipGetLastError());
   }
-  NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, n, nrhs, f.F, n, T, nrhs));
-  NMX_TRY(trsm_left<float>(ctx, 1, 0, 0, n, nrhs, f.F, n, T, nrhs));
+  NMX_TRY(mp_trsm_blocks(ctx, f, 0, nrhs, T, Y));  // Y = L^-1 T
+  NMX_TRY(mp_trsm_blocks(ctx, f, 1, nrhs, Y, T));  // T = U^-1 Y
   if (f.pivot) {
     mat_f32_add_f64<<<(unsigned)(((int64_t)n * nrhs + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, T, nrhs, D, ldd,
                                                                                           accumulate);
@@ -1421,7 +1445,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   const size_t blk = (size_t)n * nrhs;
   if (st == NMX_OK)
     st = nmx_workspace(ctx, sizeof(double) * blk + sizeof(unsigned long long) * (2 * (size_t)nrhs + 8) +
-                                sizeof(int32_t) * (size_t)nrhs + sizeof(float) * blk + 256,
+                                sizeof(int32_t) * (size_t)nrhs + 2 * sizeof(float) * blk + 512,
                        &wsp);
   if (st != NMX_OK) {
     mp_free(ctx, f);
@@ -1431,6 +1455,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   unsigned long long* norms = (unsigned long long*)(R + blk);  // [0, nrhs) |r|, [nrhs, 2 nrhs) |x|, then |A|
   int32_t* flag = (int32_t*)(norms + 2 * (size_t)nrhs + 8);
   float* T = (float*)(((uintptr_t)(flag + nrhs) + 255) & ~(uintptr_t)255);
+  float* Yb = (float*)(((uintptr_t)(T + blk) + 255) & ~(uintptr_t)255);
 
   hipEvent_t e0, e1, e2;
   hipEventCreate(&e0);
@@ -1439,7 +1464,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   hipEventRecord(e0, ctx->stream);
   st = mp_factor(ctx, f, A, lda, cfg.seed);
   hipEventRecord(e1, ctx->stream);
-  if (st == NMX_OK) st = mp_solve_mat(ctx, f, nrhs, B, ldb, X, ldx, T, 0);  // X0 = M^-1 B
+  if (st == NMX_OK) st = mp_solve_mat(ctx, f, nrhs, B, ldb, X, ldx, T, Yb, 0);  // X0 = M^-1 B
   double anorm = 0.0;
   if (st == NMX_OK) {
     zero_u64<<<1, 64, 0, ctx->stream>>>(norms + 2 * nrhs, 1);
@@ -1497,7 +1522,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
     NMX_HIP(ctx, hipMemcpyAsync(flag, hflag.data(), sizeof(int32_t) * nrhs, hipMemcpyHostToDevice, ctx->stream));
     zero_cols_kernel<<<(unsigned)((blk + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, R, nrhs, flag);
     NMX_HIP(ctx, hipGetLastError());
-    st = mp_solve_mat(ctx, f, nrhs, R, nrhs, X, ldx, T, 1);
+    st = mp_solve_mat(ctx, f, nrhs, R, nrhs, X, ldx, T, Yb, 1);
     it++;
     NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // hflag's host memory is rewritten next step
   }
