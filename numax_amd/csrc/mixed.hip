@@ -574,16 +574,31 @@ __device__ __forceinline__ void tile_fma(float s[kTrsvRows], const TileRegs& T, 
 // Sum N per-lane partials over the wavefront by recursive halving
 // (reduce-scatter, N - 1 + log2(64 / N) exchanges instead of 6 N).  Lane l
 // ends with the total of partial l / (64 / N).
+// a where the lane's bit of m is set, else b: one v_cndmask on a constant
+// lane mask (written as a plain select, `hi ? s[w + k] : s[k]` becomes an
+// indexed access and the array is moved to LDS)
+__device__ __forceinline__ float lane_sel(uint64_t m, float a, float b) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+  return r;
+}
+__host__ __device__ constexpr uint64_t lanes_with_bit(int m) {
+  return m == 32 ? 0xFFFFFFFF00000000ull
+                 : m == 16 ? 0xFFFF0000FFFF0000ull
+                           : m == 8 ? 0xFF00FF00FF00FF00ull
+                                    : m == 4 ? 0xF0F0F0F0F0F0F0F0ull : m == 2 ? 0xCCCCCCCCCCCCCCCCull : 0xAAAAAAAAAAAAAAAAull;
+}
+
 template <int N>
 __device__ __forceinline__ float wave_reduce(float s[N], int lane) {
   int m = 32;
 #pragma unroll
   for (int w = N / 2; w >= 1; w >>= 1, m >>= 1) {
-    const bool hi = (lane & m) != 0;
+    const uint64_t hm = lanes_with_bit(m);
 #pragma unroll
     for (int k = 0; k < w; k++) {
-      const float keep = hi ? s[w + k] : s[k];
-      const float send = hi ? s[k] : s[w + k];
+      const float keep = lane_sel(hm, s[w + k], s[k]);
+      const float send = lane_sel(hm, s[k], s[w + k]);
       s[k] = keep + __shfl_xor(send, m);
     }
   }
@@ -616,8 +631,14 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
   float s[kTrsvRows];
 #pragma unroll
   for (int q = 0; q < kTrsvRows; q++) s[q] = 0.f;
-  TileRegs Ia, Ib;
+  TileRegs Ia, Ib, Iv;
   const int steps = UPPER ? nblk - 1 - bi : bi;
+  // the diagonal inverse and this block's right-hand side depend on no other
+  // block: loaded first, so the end of the chain waits on neither
+  tile_load(Iv, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
+  constexpr int kSpreadRows = 64 / kTrsvRows;  // lanes holding one row's total
+  const int my_row = wave + kTrsvWaves * (lane / kSpreadRows);
+  const float my_rhs = (lane % kSpreadRows == 0 && my_row < nb) ? x[k0 + my_row] : 0.f;
   auto col_block = [&](int st) { return UPPER ? nblk - 1 - st : st; };
   auto fetch = [&](TileRegs& T, int st) {
     const int j0 = col_block(st) * TB;
@@ -658,11 +679,11 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
   // rhs = b - L_i,<i x_<i, then x_i = inv_ii rhs, stored plain and
   // published tagged
   auto finish = [&](const TileRegs& Inv) {
-    constexpr int kSpread = 64 / kTrsvRows;  // lanes holding one row's total
-    const int r = wave + kTrsvWaves * (lane / kSpread);
+    constexpr int kSpread = kSpreadRows;
+    const int r = my_row;
     {
       const float v = wave_reduce<kTrsvRows>(s, lane);
-      if (lane % kSpread == 0) rhs[r] = (r < nb) ? x[k0 + r] - v : 0.f;
+      if (lane % kSpread == 0) rhs[r] = (r < nb) ? my_rhs - v : 0.f;
     }
     __syncthreads();
 #pragma unroll
@@ -675,17 +696,11 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
                          __HIP_MEMORY_SCOPE_AGENT);
     }
   };
-  // the diagonal inverse rides in whichever slab is free (a uniform branch,
-  // so both slabs stay in registers)
   if (st < steps) {
-    tile_load(Ib, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
     take(st);
     tile_fma(s, Ia, xs[st & 1], lane);
-    finish(Ib);
-  } else {
-    tile_load(Ia, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
-    finish(Ia);
   }
+  finish(Iv);
   if (t == 0 && aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
   if (stamps && t == 0) stamps[bi] = __builtin_amdgcn_s_memrealtime();  // 100 MHz clock
 }
