@@ -103,7 +103,7 @@ struct PivKey<double> {
 // other column of the matrix right after (laswp_gather_kernel), which leaves
 // the matrix numax's full-width swaps leave (panel.mojo:404-411).
 //
-// Workgroup p of `parts` owns rows r0 + 1024 p + t + 512 i (i = 0, 1) of
+// Workgroup p of `parts` owns rows r0 + 512 ROWS p + t + 512 i (i < ROWS) of
 // its 512 threads, each row's 32 columns in registers.  The column loop stays
 // rolled (unrolled it overflows the instruction cache); the rows are rotated
 // left by one after each column so register indices stay static: at step j,
@@ -116,11 +116,13 @@ struct PivKey<double> {
 // every thread of every part spins on the words it stages -- one round trip,
 // no fences.  Slots alternate by epoch parity.
 // ---------------------------------------------------------------------------
-constexpr int kLeafRows = 2;                    // rows per thread
-constexpr int kLeafT = kLeafThreads / kLeafRows;  // 512 threads
+constexpr int kLeafT = 512;  // threads per workgroup
 constexpr int kLeafWaves = kLeafT / 64;
 
-template <typename T>
+// ROWS rows per thread: 2 for a panel one workgroup holds (up to 1024
+// rows), 1 when the panel is spread over several (half the per-column
+// vector work per CU, twice the workgroups; up to 32 x 512 rows).
+template <typename T, int ROWS>
 __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda, int n, int r0, int c0, int w,
                                                      int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
                                                      uint64_t* __restrict__ mb, uint32_t epoch0, int parts, int* err) {
@@ -151,13 +153,13 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
     info_set = (*info != 0);
   }
 
-  const int row0 = r0 + p * kLeafThreads + t;  // row i = row0 + 512 i
-  T v[kLeafRows][W];
+  const int row0 = r0 + p * (kLeafT * ROWS) + t;  // row i = row0 + 512 i
+  T v[ROWS][W];
   {
     // one descriptor at this workgroup's first row: offsets stay below 1024 rows
-    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + (int64_t)(r0 + p * kLeafThreads) * lda + c0);
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + (int64_t)(r0 + p * (kLeafT * ROWS)) * lda + c0);
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) {
+    for (int i = 0; i < ROWS; i++) {
       const bool own = row0 + kLeafT * i < n;
       const uint32_t rel = (uint32_t)(t + kLeafT * i) * (uint32_t)lda;
 #pragma unroll
@@ -177,7 +179,7 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
     Key tk = 0;
     int trow = INT_MAX;
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) {
+    for (int i = 0; i < ROWS; i++) {
       const int row = row0 + kLeafT * i;
       if (row >= gr && row < n) {
         T mag = fabs(v[i][0]);
@@ -194,7 +196,7 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
     const int wrow = (int)wave_umin(tk == wk ? (uint32_t)trow : 0xFFFFFFFFu);
     // its row values (and the diagonal row's) into LDS
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) {
+    for (int i = 0; i < ROWS; i++) {
       const int row = row0 + kLeafT * i;
       if (wk != 0 && row == wrow) {
 #pragma unroll
@@ -299,7 +301,7 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
     auto urow_at = [&](int c) -> T { return kRowRegs ? u[kRowRegs ? c : 0] : urow[c]; };
     // 7. interchange inside the leaf's columns, record the pivot
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) {
+    for (int i = 0; i < ROWS; i++) {
       const int row = row0 + kLeafT * i;
       if (wr != gr && row < n) {
         if (row == gr) {
@@ -328,7 +330,7 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
     int rem = w - j;
     asm volatile("" : "+v"(rem));
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) {
+    for (int i = 0; i < ROWS; i++) {
       const int row = row0 + kLeafT * i;
       if (row > gr && row < n) {
         const T l = v[i][0] / pivot;
@@ -344,12 +346,12 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
   }
   for (int j = w; j < W; j++) {
 #pragma unroll
-    for (int i = 0; i < kLeafRows; i++) rotate_left(v[i]);
+    for (int i = 0; i < ROWS; i++) rotate_left(v[i]);
   }
 
   if (aborted && t == 0) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
 #pragma unroll
-  for (int i = 0; i < kLeafRows; i++) {
+  for (int i = 0; i < ROWS; i++) {
     const int row = row0 + kLeafT * i;
     if (row < n) {
       T* dst = A + (int64_t)row * lda + c0;
@@ -871,7 +873,10 @@ int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w,
   const int m = n - r0;
   if (m <= 0 || w <= 0) return NMX_OK;
   if (w > kLeafW) return NMX_ERR_UNSUPPORTED;
-  int parts = ceil_div(m, kLeafThreads);
+  // one row per thread once the panel spans several workgroups (and fits 32
+  // of 512 rows), two otherwise
+  const bool one_row = m > kLeafT * 2 && m <= kLeafT * kMaxParts;
+  int parts = ceil_div(m, one_row ? kLeafT : kLeafT * 2);
   if (parts > kMaxParts) return NMX_ERR_UNSUPPORTED;  // n - r0 <= 32768 rows
   uint32_t epoch0 = 0;
   NMX_TRY(nmx_epoch_reserve(ctx, (uint32_t)w, &epoch0));
@@ -884,7 +889,10 @@ int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w,
     // A participant that is not resident makes the others' bounded waits
     // expire, which the context reports as NMX_ERR_SYNC_TIMEOUT.
     const int grid = parts == 1 ? 1 : parts * kXcds;
-    getf2_leaf<T><<<grid, kLeafT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
+    if (one_row)
+      getf2_leaf<T, 1><<<grid, kLeafT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
+    else
+      getf2_leaf<T, 2><<<grid, kLeafT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, ipiv, info, mb, epoch0, parts, err);
     NMX_HIP(ctx, hipGetLastError());
   }
   if (swap_others) return laswp_launch<T>(ctx, A, lda, 0, n, c0, c0 + w, ipiv, r0, r0 + w);
