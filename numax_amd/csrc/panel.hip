@@ -723,9 +723,13 @@ __global__ void __launch_bounds__(256) tri_apply_small(int n, int nvec, const T*
   }
 }
 
+constexpr int kBaseInverseMinVec = 1 << 30;  // substitution for every width
+
 template <typename T, int UPPER, int UNIT, int TRANS, int VEC_ROWS>
 void launch_base(nmx_ctx* ctx, int n, int nvec, const T* Tm, int ldt, T* B, int ldb) {
-  if (nvec > 128 && ctx->tri_inv) {
+  // the explicit inverse only pays off for very wide right-hand sides: the
+  // register-resident substitution above runs 64 vectors per wavefront
+  if (nvec > kBaseInverseMinVec && ctx->tri_inv) {
     T* inv = (T*)ctx->tri_inv;
     ProfScope ps(ctx, K_TRSM, 2.0 * n * (double)n * nvec, sizeof(T) * (2.0 * n * nvec + 64.0 * 64));
     tri_inv_small<T, UPPER, UNIT, TRANS><<<1, 1024, 0, ctx->stream>>>(n, Tm, ldt, inv);
