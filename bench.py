@@ -41,10 +41,9 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
 # n = 16384 for the LU paths, the 20,000-system batch for the batched leg.
 # Keyed by (kernel class, n or batch count); another size reports null.
 MEASURED_TRAFFIC = {
-    ("getf2_leaf", 16384): (2.11 + 9.86) * 2 ** 20,        # mixed-precision path, FP32 leaf
-    ("mp_gemm", 16384): (598.59 + 338.67) * 2 ** 20,
-    ("batched_solve", 20000): (11364.06 + 2317.26 + 7290.70 + 1534.23 + 5119.97 + 1010.50 + 3374.91 + 708.27
-                               + 1341.29 + 295.09 + 973.99 + 214.54 + 86.90 + 23.30 + 54.36 + 21.28) * 2 ** 20,
+    ("getf2_leaf", 16384): 13.55 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
+    ("mp_gemm", 16384): 937.36 * 2 ** 20,
+    ("batched_solve", 20000): 8 * 4461.96 * 2 ** 20,  # the eight class launches of one call
 }
 
 
