@@ -322,7 +322,8 @@ __global__ void convert_kernel(int rows, int cols, const float* __restrict__ src
 // ---------------------------------------------------------------------------
 // C (FP32, in place) -= (A . B^T) / (sA sB) with A m x k, B n x k in FP16 /
 // BF16, FP32 accumulation on v_mfma_f32_32x32x16_{f16,bf16}.  128x128 C tile
-// per 256-thread workgroup, 64x64 per wavefront (2x2 MFMA tiles); the C tile
+// per 512-thread workgroup, 64x32 per wavefront (2x1 MFMA tiles, so up to three
+// wavefronts fit a SIMD's registers and hide the slab loads); the C tile
 // is prefetched into registers at the start so its HBM read overlaps the
 // MFMA work.  SPLIT adds 2^-11 (A_hi B_lo^T + A_lo B_hi^T).
 // ---------------------------------------------------------------------------
@@ -342,9 +343,10 @@ struct Mma<1> {
 };
 
 constexpr int MT = 128, KT = 32, KP = 8;  // C tile, K slab, LDS pad (halves)
+constexpr int kMpThreads = 512;           // 8 wavefronts, each a 64 x 32 piece of the C tile
 
 template <int BF, int SPLIT>
-__global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const typename Lo<BF>::t* __restrict__ Ah,
+__global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K, const typename Lo<BF>::t* __restrict__ Ah,
                                                       const typename Lo<BF>::t* __restrict__ Al,
                                                       const typename Lo<BF>::t* __restrict__ Bh,
                                                       const typename Lo<BF>::t* __restrict__ Bl,
@@ -370,47 +372,47 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
   const int m0 = (first + in % gm) * MT, n0 = (in / gm) * MT;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 wavefronts of 64 x 32
 
   // prefetch this lane's C values (the MFMA output layout)
-  float cpre[2][2][16];
+  float cpre[2][1][16];
   {
     const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-      for (int j = 0; j < 2; j++)
+      for (int j = 0; j < 1; j++)
 #pragma unroll
         for (int e = 0; e < 16; e++) {
           const int lr = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-          const int lc = wn * 64 + j * 32 + (lane & 31);
+          const int lc = wn * 32 + j * 32 + (lane & 31);
           const uint32_t off =
               ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * 4u : kBufOob;
           cpre[i][j][e] = buf_load<float>(rc, off);
         }
   }
 
-  f16v acc[2][2], accx[2][2];
+  f16v acc[2][1], accx[2][1];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < 1; j++) {
       acc[i][j] = f16v{};
       accx[i][j] = f16v{};
     }
 
   // global -> registers -> LDS, double-buffered: slab s+1 is in flight while
   // slab s feeds the MFMAs.  Each slab is 128 rows x 32 K values per operand
-  // = 512 chunks of 8, two per thread.
+  // = 512 chunks of 8, one per thread.
   const __amdgpu_buffer_rsrc_t rAh = buf_rsrc(Ah + (int64_t)m0 * K);
   const __amdgpu_buffer_rsrc_t rBh = buf_rsrc(Bh + (int64_t)n0 * K);
   const __amdgpu_buffer_rsrc_t rAl = buf_rsrc((SPLIT ? Al : Ah) + (int64_t)m0 * K);
   const __amdgpu_buffer_rsrc_t rBl = buf_rsrc((SPLIT ? Bl : Bh) + (int64_t)n0 * K);
-  v8 pa[NOP][2], pb[NOP][2];
+  v8 pa[NOP][1], pb[NOP][1];
   auto load_slab = [&](int k0) {
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const int chunk = t + c * 256;
+    for (int c = 0; c < 1; c++) {
+      const int chunk = t + c * kMpThreads;
       const int r = chunk >> 2, kk = (chunk & 3) * 8;
       const int gk = k0 + kk;
       const uint32_t off = ((uint32_t)r * (uint32_t)K + (uint32_t)gk) * 2u;
@@ -428,8 +430,8 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
 #pragma unroll
     for (int op = 0; op < NOP; op++)
 #pragma unroll
-      for (int c = 0; c < 2; c++) {
-        const int chunk = t + c * 256;
+      for (int c = 0; c < 1; c++) {
+        const int chunk = t + c * kMpThreads;
         const int r = chunk >> 2, kk = (chunk & 3) * 8;
         *(v8*)&As[buf][op][r][kk] = pa[op][c];
         *(v8*)&Bs[buf][op][r][kk] = pb[op][c];
@@ -445,20 +447,21 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
 #pragma unroll
     for (int ks = 0; ks < KT; ks += 16) {
       const int ko = ks + 8 * (lane >> 5);
-      v8 a0[2], b0[2], a1[2], b1[2];
+      v8 a0[2], b0[1], a1[2], b1[1];
 #pragma unroll
       for (int i = 0; i < 2; i++) {
         a0[i] = *(const v8*)&As[buf][0][wm * 64 + i * 32 + (lane & 31)][ko];
-        b0[i] = *(const v8*)&Bs[buf][0][wn * 64 + i * 32 + (lane & 31)][ko];
-        if (SPLIT) {
-          a1[i] = *(const v8*)&As[buf][NOP - 1][wm * 64 + i * 32 + (lane & 31)][ko];
-          b1[i] = *(const v8*)&Bs[buf][NOP - 1][wn * 64 + i * 32 + (lane & 31)][ko];
-        }
+        if (SPLIT) a1[i] = *(const v8*)&As[buf][NOP - 1][wm * 64 + i * 32 + (lane & 31)][ko];
+      }
+#pragma unroll
+      for (int j = 0; j < 1; j++) {
+        b0[j] = *(const v8*)&Bs[buf][0][wn * 32 + j * 32 + (lane & 31)][ko];
+        if (SPLIT) b1[j] = *(const v8*)&Bs[buf][NOP - 1][wn * 32 + j * 32 + (lane & 31)][ko];
       }
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < 1; j++) {
           acc[i][j] = Mma<BF>::op(a0[i], b0[j], acc[i][j]);
           if (SPLIT) {
             accx[i][j] = Mma<BF>::op(a0[i], b1[j], accx[i][j]);
@@ -476,11 +479,11 @@ __global__ void __launch_bounds__(256) mp_gemm_kernel(int M, int N, int K, const
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < 1; j++)
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         const int row = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+        const int col = n0 + wn * 32 + j * 32 + (lane & 31);
         float prod = acc[i][j][e];
         if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
         const float out = cpre[i][j][e] - prod * inv;
@@ -927,7 +930,7 @@ static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   {
     ProfScope ps(ctx, K_MP_GEMM, 2.0 * m * (double)m * nb * (SPLIT ? 3 : 1),
                  8.0 * (double)m * m + (double)sizeof(lt) * 2.0 * m * nb * (SPLIT ? 2 : 1));
-    mp_gemm_kernel<BF, SPLIT><<<grid, 256, 0, ctx->stream>>>(m, m, nb, Ah, Al, Bh, Bl, f.maxbits, f.maxbits + 1,
+    mp_gemm_kernel<BF, SPLIT><<<grid, kMpThreads, 0, ctx->stream>>>(m, m, nb, Ah, Al, Bh, Bl, f.maxbits, f.maxbits + 1,
                                                               F + (int64_t)(k + nb) * n + k + nb, n, tiles, tiles);
     NMX_HIP(ctx, hipGetLastError());
   }
