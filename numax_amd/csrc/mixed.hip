@@ -322,10 +322,10 @@ __global__ void convert_kernel(int rows, int cols, const float* __restrict__ src
 // ---------------------------------------------------------------------------
 // C (FP32, in place) -= (A . B^T) / (sA sB) with A m x k, B n x k in FP16 /
 // BF16, FP32 accumulation on v_mfma_f32_32x32x16_{f16,bf16}.  128x128 C tile
-// per 512-thread workgroup, 64x32 per wavefront (2x1 MFMA tiles, so up to three
+// per 512-thread workgroup, 64x32 per wavefront (2x1 MFMA tiles, so four
 // wavefronts fit a SIMD's registers and hide the slab loads); the C tile
-// is prefetched into registers at the start so its HBM read overlaps the
-// MFMA work.  SPLIT adds 2^-11 (A_hi B_lo^T + A_lo B_hi^T).
+// is read after the products, its latency hidden by the other wavefronts.
+// SPLIT adds 2^-11 (A_hi B_lo^T + A_lo B_hi^T).
 // ---------------------------------------------------------------------------
 template <int BF>
 struct Mma;
@@ -373,24 +373,6 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 wavefronts of 64 x 32
-
-  // prefetch this lane's C values (the MFMA output layout)
-  float cpre[2][1][16];
-  {
-    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 1; j++)
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-          const int lr = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-          const int lc = wn * 32 + j * 32 + (lane & 31);
-          const uint32_t off =
-              ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * 4u : kBufOob;
-          cpre[i][j][e] = buf_load<float>(rc, off);
-        }
-  }
 
   f16v acc[2][1], accx[2][1];
 #pragma unroll
@@ -473,6 +455,25 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
       store_slab(buf ^ 1);
       __syncthreads();
     }
+  }
+
+  // this lane's C values (the MFMA output layout), read after the products:
+  // keeping them live through the K loop would cost a wavefront per SIMD
+  float cpre[2][1][16];
+  {
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 1; j++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          const int lr = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+          const int lc = wn * 32 + j * 32 + (lane & 31);
+          const uint32_t off =
+              ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * 4u : kBufOob;
+          cpre[i][j][e] = buf_load<float>(rc, off);
+        }
   }
 
   const float inv = 1.0f / (pow2_scale(*maxA) * pow2_scale(*maxB));
