@@ -41,9 +41,9 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
 # n = 16384 for the LU paths, the 20,000-system batch for the batched leg.
 # Keyed by (kernel class, n or batch count); another size reports null.
 MEASURED_TRAFFIC = {
-    ("getf2_leaf", 16384): 13.55 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
-    ("mp_gemm", 16384): 937.36 * 2 ** 20,
-    ("batched_solve", 20000): 8 * 4461.96 * 2 ** 20,  # the eight class launches of one call
+    ("getf2_leaf", 16384): 13.52 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
+    ("mp_gemm", 16384): 823.64 * 2 ** 20,
+    ("batched_solve", 20000): 8 * 4466.57 * 2 ** 20,  # the eight class launches of one call
 }
 
 
