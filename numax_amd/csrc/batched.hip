@@ -255,17 +255,23 @@ __device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k
   const int moves = __popcll(mm);
   wave_sync();
   if (moves == 0) return;
+  // lane e holds move e; each move's rows reach every lane by v_readlane (an
+  // LDS read per move put one round trip ahead of each load).  All lanes stay
+  // active: a column outside the range loads column 0 and stores nothing.
+  static_assert(2 * kNB == 64, "one move per lane");
+  const int my_src = S.mv_src[lane < moves ? lane : moves - 1];
+  const int my_dst = S.mv_dst[lane < moves ? lane : moves - 1];
 #pragma unroll 1
   for (int s = 0; s < Q; s++) {
     const int c = lane + 64 * s;
-    if (c < n && (c < k || c >= k + nb)) {
-      float vals[2 * kNB];
+    const bool ok = c < n && (c < k || c >= k + nb);
+    const int cl = ok ? c : 0;
+    float vals[2 * kNB];
 #pragma unroll
-      for (int e = 0; e < 2 * kNB; e++) vals[e] = F[(int64_t)S.mv_src[e < moves ? e : moves - 1] * n + c];
+    for (int e = 0; e < 2 * kNB; e++) vals[e] = F[(int64_t)__builtin_amdgcn_readlane(my_src, e) * n + cl];
 #pragma unroll
-      for (int e = 0; e < 2 * kNB; e++)
-        if (e < moves) F[(int64_t)S.mv_dst[e] * n + c] = vals[e];
-    }
+    for (int e = 0; e < 2 * kNB; e++)
+      if (ok && e < moves) F[(int64_t)__builtin_amdgcn_readlane(my_dst, e) * n + cl] = vals[e];
   }
   wave_sync();
 }

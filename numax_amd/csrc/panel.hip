@@ -407,19 +407,30 @@ __global__ void __launch_bounds__(256) laswp_gather_kernel(T* __restrict__ A, in
   }
   __syncthreads();
   const int moves = mv_count;
+  if (moves == 0) return;
+  // lane e of each wavefront takes move e out of LDS (one read), and every
+  // move's rows then come to the whole wavefront by v_readlane: read per
+  // move, the list cost one LDS round trip ahead of each load.  Every lane
+  // stays active to the end (v_readlane reads lanes the branch would have
+  // turned off); lanes past the last column load from the first one and
+  // store nothing.
+  static_assert(2 * kLeafW == 64, "one move per lane");
+  const int lane = t & 63;
+  const int my_src = mv_src[lane < moves ? lane : moves - 1];
+  const int my_dst = mv_dst[lane < moves ? lane : moves - 1];
   const int i = blockIdx.x * blockDim.x + t;
-  if (moves == 0 || i >= ncols) return;
-  int c = col_lo + i;
+  const bool act = i < ncols;
+  int c = col_lo + (act ? i : 0);
   if (c >= skip_lo) c += skip_w;
   T vals[2 * kLeafW];
 #pragma unroll
   for (int e = 0; e < 2 * kLeafW; e++) {
     // unconditional loads (a clamped, valid row) so all are in flight together
-    vals[e] = A[(int64_t)mv_src[e < moves ? e : moves - 1] * lda + c];
+    vals[e] = A[(int64_t)__builtin_amdgcn_readlane(my_src, e) * lda + c];
   }
 #pragma unroll
   for (int e = 0; e < 2 * kLeafW; e++)
-    if (e < moves) A[(int64_t)mv_dst[e] * lda + c] = vals[e];
+    if (act && e < moves) A[(int64_t)__builtin_amdgcn_readlane(my_dst, e) * lda + c] = vals[e];
 }
 
 // Replay interchanges k1..k2-1 (in order) over the columns [col_lo, col_hi)
@@ -504,7 +515,7 @@ __global__ void __launch_bounds__(256) gather_rows_kernel(int n, int ncols, cons
 // vectors live: the columns of B (left solves) or its rows (the Cholesky
 // panel X L^-T).
 template <typename T, int NB, int UPPER, int UNIT, int TRANS, int VEC_ROWS>
-__global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __restrict__ Tm, int ldt, T* __restrict__ B,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) trsm_base(int n, int nvec, const T* __restrict__ Tm, int ldt, T* __restrict__ B,
                                                 int ldb) {
   __shared__ T Tc[NB][NB + 16 / sizeof(T)];  // Tc[j][i] = op(T)[i][j]
   const int t = threadIdx.x;
@@ -543,9 +554,17 @@ __global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __rest
   }
   // rows and columns past n are the identity and x is 0 there: they change
   // nothing.  Each step's column is addressed through an index made opaque
-  // after x_j is known, which keeps its LDS reads in that step and after the
-  // division (hoisted, they would hold up to NB^2/2 registers).
-  if (!UPPER) {
+  // on a value of the step before, so the compiler cannot hoist the whole
+  // triangle into registers (NB^2/2 of them), and a scheduling barrier makes
+  // the step's LDS reads go out together: left to itself the compiler issues
+  // them one pair at a time between the products, a full LDS round trip per
+  // pair (2.3x slower at NB = 64).  FP32 reads step j's column during step
+  // j-1, right after the product that finishes x_j and before the rest, so
+  // the latency hides under those products; FP64 (twice the registers) reads
+  // it at the start of step j, which measured faster there.  Every x_i takes
+  // the same products in the same order as plain substitution.
+  constexpr bool kAhead = sizeof(T) == 4;
+  if (!UPPER && !kAhead) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
@@ -553,10 +572,14 @@ __global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __rest
       int jj = j;
       asm volatile("" : "+s"(jj) : "v"(xj));
       const T* col = Tc[jj];
+      T cv[NB];
 #pragma unroll
-      for (int i = j + 1; i < NB; i++) x[i] = x[i] - col[i] * xj;
+      for (int i = j + 1; i < NB; i++) cv[i] = col[i];
+      __builtin_amdgcn_sched_barrier(0);  // the step's reads all go out before its products
+#pragma unroll
+      for (int i = j + 1; i < NB; i++) x[i] = x[i] - cv[i] * xj;
     }
-  } else {
+  } else if (UPPER && !kAhead) {
 #pragma unroll
     for (int j = NB - 1; j >= 0; j--) {
       const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
@@ -564,8 +587,70 @@ __global__ void __launch_bounds__(64) trsm_base(int n, int nvec, const T* __rest
       int jj = j;
       asm volatile("" : "+s"(jj) : "v"(xj));
       const T* col = Tc[jj];
+      T cv[NB];
 #pragma unroll
-      for (int i = 0; i < j; i++) x[i] = x[i] - col[i] * xj;
+      for (int i = 0; i < j; i++) cv[i] = col[i];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < j; i++) x[i] = x[i] - cv[i] * xj;
+    }
+  } else if (!UPPER) {
+    T cv[NB];  // the current step's column, read one step ahead
+    {
+      int jj = 0;
+      asm volatile("" : "+s"(jj));
+      const T* col = Tc[jj];
+#pragma unroll
+      for (int i = 1; i < NB; i++) cv[i] = col[i];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
+      x[j] = xj;
+      if (j + 1 < NB) {
+        __builtin_amdgcn_sched_barrier(0);
+        x[j + 1] = x[j + 1] - cv[j + 1] * xj;  // the next step's unknown first
+        T cn[NB];
+        int jj = j + 1;
+        asm volatile("" : "+s"(jj) : "v"(x[j + 1]));
+        const T* col = Tc[jj];
+#pragma unroll
+        for (int i = j + 2; i < NB; i++) cn[i] = col[i];
+        __builtin_amdgcn_sched_barrier(0);  // the reads go out before this step's products
+#pragma unroll
+        for (int i = j + 2; i < NB; i++) x[i] = x[i] - cv[i] * xj;
+#pragma unroll
+        for (int i = j + 2; i < NB; i++) cv[i] = cn[i];
+      }
+    }
+  } else {
+    T cv[NB];
+    {
+      int jj = NB - 1;
+      asm volatile("" : "+s"(jj));
+      const T* col = Tc[jj];
+#pragma unroll
+      for (int i = 0; i < NB - 1; i++) cv[i] = col[i];
+    }
+#pragma unroll
+    for (int j = NB - 1; j >= 0; j--) {
+      const T xj = UNIT ? x[j] : x[j] / Tc[j][j];
+      x[j] = xj;
+      if (j > 0) {
+        __builtin_amdgcn_sched_barrier(0);
+        x[j - 1] = x[j - 1] - cv[j - 1] * xj;
+        T cn[NB];
+        int jj = j - 1;
+        asm volatile("" : "+s"(jj) : "v"(x[j - 1]));
+        const T* col = Tc[jj];
+#pragma unroll
+        for (int i = 0; i < j - 1; i++) cn[i] = col[i];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < j - 1; i++) x[i] = x[i] - cv[i] * xj;
+#pragma unroll
+        for (int i = 0; i < j - 1; i++) cv[i] = cn[i];
+      }
     }
   }
   // the store offsets are recomputed from an opaque lane index: shared with
