@@ -8,8 +8,9 @@
 //
 // Tiling for CDNA4: a 256-thread workgroup owns a 128x128 tile of C, four
 // 64-wide wavefronts in a 2x2 arrangement each owning 64x64 = 4x4 MFMA
-// 16x16 tiles.  K advances 16 at a time through double-buffered LDS with the
-// next K slab prefetched into registers while the current one is consumed.
+// 16x16 tiles (64x64 tiles of 2x2 per wavefront when 128x128 ones would not
+// fill the device).  K advances 16 at a time through double-buffered LDS with
+// the next K slab prefetched into registers while the current one is consumed.
 // FP64 runs v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32.
 //
 // Tile order is XCD-aware: workgroup b runs on XCD b % 8, so each XCD is
@@ -44,20 +45,24 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
 
-constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+constexpr int BM = 128, BK = 16, PAD = 4;  // the large tile, K slab, LDS row pad
 
 // Two workgroups per CU (two wavefronts per SIMD): the FP64 accumulators then
 // live in the 256 unified registers with a few epilogue-only spills, and the
 // second wave hides the LDS and global latency of the first (1.5x over one
 // wave per SIMD on the 16128x16128x256 trailing update).  Two double-buffered
 // FP64 tiles take 68 KB of LDS each, which is what 160 KB holds.
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int TILE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T beta,
             T* __restrict__ C, int ldc, int tilesM, int tilesN, int lower_only) {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[2][BK][BM + PAD];
-  __shared__ T Bs[2][BK][BN + PAD];
+  // TILE x TILE of C per workgroup; each wavefront owns WT x WT = FR x FR
+  // MFMA tiles; each thread stages LPT elements of each K slab
+  constexpr int TM = TILE, TN = TILE, WT = TILE / 2, FR = WT / 16, LPT = TILE * BK / 256;
+  static_assert(FR >= 1 && LPT >= 1 && BK % LPT == 0, "tile shape");
+  __shared__ T As[2][BK][TM + PAD];
+  __shared__ T Bs[2][BK][TN + PAD];
 
   // ---- tile assignment: contiguous chunk per XCD, grouped rows inside ----
   const int total = lower_only ? tilesM * (tilesM + 1) / 2 : tilesM * tilesN;
@@ -82,57 +87,57 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
     tm = first + in % gm;
     tn = in / gm;
   }
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * TM, n0 = tn * TN;
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  acc_t acc[4][4];
+  acc_t acc[FR][FR];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < FR; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < FR; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[8], rb[8];
+  T ra[LPT], rb[LPT];
   // Slab loads through buffer descriptors: an element outside the operand
   // gets an offset past the descriptor's range and reads as 0, so edge and
   // interior tiles run the same branch-free code with all loads in flight.
   constexpr uint32_t E = sizeof(T);
   auto load_tiles = [&](int k0) {
     if (!TA) {
-      const int r = t >> 1, kh = (t & 1) * 8, gm = m0 + r;
+      const int r = t / (BK / LPT), kh = (t % (BK / LPT)) * LPT, gm = m0 + r;
       const __amdgpu_buffer_rsrc_t ra_ = buf_rsrc(A + (int64_t)m0 * lda);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < LPT; i++) {
         const int gk = k0 + kh + i;
         const uint32_t off = ((gm < M) & (gk < K)) ? ((uint32_t)r * (uint32_t)lda + (uint32_t)gk) * E : kBufOob;
         ra[i] = buf_load<T>(ra_, off);
       }
     } else {
-      const int kk = t >> 4, mc = (t & 15) * 8, gk = k0 + kk;
+      const int kk = t / (TM / LPT), mc = (t % (TM / LPT)) * LPT, gk = k0 + kk;
       const __amdgpu_buffer_rsrc_t ra_ = buf_rsrc(A + (int64_t)k0 * lda + m0);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < LPT; i++) {
         const int gm = m0 + mc + i;
         const uint32_t off = ((gm < M) & (gk < K)) ? ((uint32_t)kk * (uint32_t)lda + (uint32_t)(mc + i)) * E : kBufOob;
         ra[i] = buf_load<T>(ra_, off);
       }
     }
     if (!TB) {
-      const int kk = t >> 4, nc = (t & 15) * 8, gk = k0 + kk;
+      const int kk = t / (TN / LPT), nc = (t % (TN / LPT)) * LPT, gk = k0 + kk;
       const __amdgpu_buffer_rsrc_t rb_ = buf_rsrc(B + (int64_t)k0 * ldb + n0);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < LPT; i++) {
         const int gn = n0 + nc + i;
         const uint32_t off = ((gn < N) & (gk < K)) ? ((uint32_t)kk * (uint32_t)ldb + (uint32_t)(nc + i)) * E : kBufOob;
         rb[i] = buf_load<T>(rb_, off);
       }
     } else {
-      const int r = t >> 1, kh = (t & 1) * 8, gn = n0 + r;
+      const int r = t / (BK / LPT), kh = (t % (BK / LPT)) * LPT, gn = n0 + r;
       const __amdgpu_buffer_rsrc_t rb_ = buf_rsrc(B + (int64_t)n0 * ldb);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < LPT; i++) {
         const int gk = k0 + kh + i;
         const uint32_t off = ((gn < N) & (gk < K)) ? ((uint32_t)r * (uint32_t)ldb + (uint32_t)gk) * E : kBufOob;
         rb[i] = buf_load<T>(rb_, off);
@@ -141,22 +146,22 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
   };
   auto store_tiles = [&](int buf) {
     if (!TA) {
-      int r = t >> 1, kh = (t & 1) * 8;
+      int r = t / (BK / LPT), kh = (t % (BK / LPT)) * LPT;
 #pragma unroll
-      for (int i = 0; i < 8; i++) As[buf][kh + i][r] = ra[i];
+      for (int i = 0; i < LPT; i++) As[buf][kh + i][r] = ra[i];
     } else {
-      int kk = t >> 4, mc = (t & 15) * 8;
+      int kk = t / (TM / LPT), mc = (t % (TM / LPT)) * LPT;
 #pragma unroll
-      for (int i = 0; i < 8; i++) As[buf][kk][mc + i] = ra[i];
+      for (int i = 0; i < LPT; i++) As[buf][kk][mc + i] = ra[i];
     }
     if (!TB) {
-      int kk = t >> 4, nc = (t & 15) * 8;
+      int kk = t / (TN / LPT), nc = (t % (TN / LPT)) * LPT;
 #pragma unroll
-      for (int i = 0; i < 8; i++) Bs[buf][kk][nc + i] = rb[i];
+      for (int i = 0; i < LPT; i++) Bs[buf][kk][nc + i] = rb[i];
     } else {
-      int r = t >> 1, kh = (t & 1) * 8;
+      int r = t / (BK / LPT), kh = (t % (BK / LPT)) * LPT;
 #pragma unroll
-      for (int i = 0; i < 8; i++) Bs[buf][kh + i][r] = rb[i];
+      for (int i = 0; i < LPT; i++) Bs[buf][kh + i][r] = rb[i];
     }
   };
 
@@ -168,11 +173,11 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
     const int buf = s & 1;
     if (s + 1 < ksteps) load_tiles((s + 1) * BK);
     // fragments for k-step kq+1 are read while kq's MFMAs issue
-    T af[2][4], bf[2][4];
+    T af[2][FR], bf[2][FR];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      af[0][i] = As[buf][lane >> 4][wm * 64 + i * 16 + (lane & 15)];
-      bf[0][i] = Bs[buf][lane >> 4][wn * 64 + i * 16 + (lane & 15)];
+    for (int i = 0; i < FR; i++) {
+      af[0][i] = As[buf][lane >> 4][wm * WT + i * 16 + (lane & 15)];
+      bf[0][i] = Bs[buf][lane >> 4][wn * WT + i * 16 + (lane & 15)];
     }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; kq++) {
@@ -180,15 +185,15 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
       if (kq + 1 < BK / 4) {
         const int kr = (kq + 1) * 4 + (lane >> 4);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          af[cur ^ 1][i] = As[buf][kr][wm * 64 + i * 16 + (lane & 15)];
-          bf[cur ^ 1][i] = Bs[buf][kr][wn * 64 + i * 16 + (lane & 15)];
+        for (int i = 0; i < FR; i++) {
+          af[cur ^ 1][i] = As[buf][kr][wm * WT + i * 16 + (lane & 15)];
+          bf[cur ^ 1][i] = Bs[buf][kr][wn * WT + i * 16 + (lane & 15)];
         }
       }
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < FR; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = Mfma<T>::mma(af[cur][i], bf[cur][j], acc[i][j]);
+        for (int j = 0; j < FR; j++) acc[i][j] = Mfma<T>::mma(af[cur][i], bf[cur][j], acc[i][j]);
     }
     if (s + 1 < ksteps) {
       store_tiles(buf ^ 1);
@@ -198,31 +203,31 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
 
   // ---- epilogue straight into the strided C: every C load is issued
   // before any is used (buffer loads, out-of-range lanes read 0)
-  T cv[4][4][4];
+  T cv[FR][FR][4];
   const bool use_c = beta != T(0);
   if (use_c) {
     const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (int64_t)m0 * ldc + n0);
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < FR; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++)
+      for (int j = 0; j < FR; j++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          const int lr = wm * 64 + i * 16 + Mfma<T>::row(lane, e);
-          const int lc = wn * 64 + j * 16 + (lane & 15);
+          const int lr = wm * WT + i * 16 + Mfma<T>::row(lane, e);
+          const int lc = wn * WT + j * 16 + (lane & 15);
           const uint32_t off =
               ((m0 + lr < M) & (n0 + lc < N)) ? ((uint32_t)lr * (uint32_t)ldc + (uint32_t)lc) * E : kBufOob;
           cv[i][j][e] = buf_load<T>(rc, off);
         }
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < FR; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    for (int j = 0; j < FR; j++) {
+      const int col = n0 + wn * WT + j * 16 + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e);
+        const int row = m0 + wm * WT + i * 16 + Mfma<T>::row(lane, e);
         T v = alpha * acc[i][j][e];
         if (use_c) v += beta * cv[i][j][e];
         if ((row < M) & (col < N)) C[(int64_t)row * ldc + col] = v;
@@ -253,20 +258,39 @@ int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alp
     return NMX_OK;
   }
   if (lower_only && m != n) return NMX_ERR_UNSUPPORTED;
-  int tilesM = ceil_div(m, BM), tilesN = ceil_div(n, BN);
-  int total = lower_only ? tilesM * (tilesM + 1) / 2 : tilesM * tilesN;
-  int grid = ceil_div(total, kXcds) * kXcds;
+  // 128x128 tiles unless they would leave the device under-filled (fewer
+  // than two per CU, the in-panel and late trailing updates): 64x64 tiles
+  // then quadruple the workgroups and quarter each one's MFMA chain.  The
+  // k order of every C element is the same either way.
+  auto tiles = [&](int tile) {
+    const int tm = ceil_div(m, tile), tn = ceil_div(n, tile);
+    return lower_only ? tm * (tm + 1) / 2 : tm * tn;
+  };
+  const int tile = tiles(BM) < 2 * ctx->num_cus ? 64 : BM;
+  const int tilesM = ceil_div(m, tile), tilesN = ceil_div(n, tile);
+  const int grid = ceil_div(tiles(tile), kXcds) * kXcds;
   dim3 g(grid), blk(256);
   const double flops = 2.0 * m * (double)n * k * (lower_only ? 0.5 : 1.0);
   ProfScope ps(ctx, K_GEMM, flops, (double)sizeof(T) * ((double)m * k + (double)k * n + 2.0 * m * n));
-  if (!transa && !transb)
-    gemm_kernel<T, 0, 0><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
-  else if (!transa && transb)
-    gemm_kernel<T, 0, 1><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
-  else if (transa && !transb)
-    gemm_kernel<T, 1, 0><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+  auto run = [&](auto tile_tag) {
+    constexpr int TL = decltype(tile_tag)::value;
+    if (!transa && !transb)
+      gemm_kernel<T, 0, 0, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
+                                                            lower_only);
+    else if (!transa && transb)
+      gemm_kernel<T, 0, 1, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
+                                                            lower_only);
+    else if (transa && !transb)
+      gemm_kernel<T, 1, 0, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
+                                                            lower_only);
+    else
+      gemm_kernel<T, 1, 1, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
+                                                            lower_only);
+  };
+  if (tile == 64)
+    run(std::integral_constant<int, 64>());
   else
-    gemm_kernel<T, 1, 1><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN, lower_only);
+    run(std::integral_constant<int, BM>());
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
