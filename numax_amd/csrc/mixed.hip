@@ -715,6 +715,7 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
 // max |r|, max |x| and (with rowabs) max row sum |A| of the stopping test,
 // one atomic per workgroup each.
 constexpr int kResidualBlocks = 1024;
+constexpr int kResidualChunks = 16;
 __global__ void __launch_bounds__(256) residual_kernel(int n, const double* __restrict__ A, int lda,
                                                        const double* __restrict__ x, const double* __restrict__ b,
                                                        double* __restrict__ r, double* __restrict__ rowabs,
@@ -727,18 +728,19 @@ __global__ void __launch_bounds__(256) residual_kernel(int n, const double* __re
   for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
     const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)row * lda);
     double s = 0.0, sa = 0.0;
-    // 8 column chunks of 64 in flight per step; past-the-end reads are 0
-    for (int c0 = 0; c0 < n; c0 += 512) {
-      double av[8], xv[8];
+    // kResidualChunks column chunks of 64 in flight per step (8 KB of the
+    // row per wavefront); past-the-end reads are 0
+    for (int c0 = 0; c0 < n; c0 += 64 * kResidualChunks) {
+      double av[kResidualChunks], xv[kResidualChunks];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < kResidualChunks; u++) {
         const int c = c0 + u * 64 + lane;
         const uint32_t off = c < n ? (uint32_t)c * 8u : kBufOob;
         av[u] = buf_load<double>(ra, off);
         xv[u] = buf_load<double>(rx, off);
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < kResidualChunks; u++) {
         s += av[u] * xv[u];
         sa += fabs(av[u]);
       }
