@@ -36,6 +36,31 @@ def test_gemm_matches_numpy(dtype, transa, transb, m, n, k):
     assert np.max(np.abs(out - ref)) <= tol * max(1.0, np.max(np.abs(ref))) * np.sqrt(k)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("transa,transb", [(0, 0), (1, 1)])
+def test_gemm_tile_shapes_agree_bitwise(dtype, transa, transb):
+    # a 2944 x 2944 update is 23 x 23 = 529 tiles of 128 x 128 (the large-tile
+    # path); its leading 384 x 320 block on its own is too small to fill the
+    # device with those and takes 64 x 64 tiles.  Every element sums its k
+    # slabs in the same order either way, so the two agree bit for bit.
+    m = n = 2944
+    k = 72
+    rng = np.random.default_rng(2944 + transa)
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    a = rng.standard_normal((k, m) if transa else (m, k)).astype(npd)
+    b = rng.standard_normal((n, k) if transb else (k, n)).astype(npd)
+    c = rng.standard_normal((m, n)).astype(npd)
+    full = _np(nl.gemm(a, b, c, alpha=-1.0, beta=1.0, transa=bool(transa), transb=bool(transb)))
+    a_s = np.ascontiguousarray(a[:, :384] if transa else a[:384])
+    b_s = np.ascontiguousarray(b[:320] if transb else b[:, :320])
+    part = _np(nl.gemm(a_s, b_s, np.ascontiguousarray(c[:384, :320]), alpha=-1.0, beta=1.0, transa=bool(transa),
+                       transb=bool(transb)))
+    assert np.array_equal(full[:384, :320], part)
+    ref = c.astype(np.float64) - (a.T if transa else a).astype(np.float64) @ (b.T if transb else b).astype(np.float64)
+    tol = 1e-12 if dtype == torch.float64 else 2e-4
+    assert np.max(np.abs(full - ref)) <= tol * max(1.0, np.max(np.abs(ref))) * np.sqrt(k)
+
+
 @pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 300])
 @pytest.mark.parametrize("upper,unit,trans", [(0, 0, 0), (0, 1, 0), (1, 0, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1)])
 def test_solve_triangular_matches_oracle(n, upper, unit, trans):
