@@ -1,7 +1,7 @@
 """Benchmark: dense LU solve on one MI355X at n = 16384.
 
-Metric: solves/s + TFLOP/s at FP64 backward error, mixed-
-precision LU-IR vs plain FP64 LU, n = 16384.
+Metric: solves/s + TFLOP/s at FP64 backward error, mixed-precision LU-IR vs
+plain FP64 LU, n = 16384.
 
 A step is one solve of A x = b from scratch: copy A into the work matrix
 (numax's lu_factor copies its input too), factor, solve, and -- for the
@@ -242,8 +242,8 @@ def main() -> int:
                          "counter-collection runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build's path runs on a single MI355X "
-              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the factor-and-solve path runs on a single MI355X "
+              "(how it would shard across GPUs is not implemented); run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch
