@@ -3,8 +3,9 @@
 ctypes bindings over ``oracle/lib/libnmx_oracle.so``, the C restatement of
 numax's float64 host algorithms (``oracle/nmx_oracle.c`` cites the reference
 file:line of every routine).  Only ``tests/``, ``__graft_entry__.smoke()`` and
-``bench.py``'s ``cpu_baseline`` leg import this package, and only as the checker
-or as the CPU baseline.  Nothing in ``numax_amd`` imports it.
+``bench.py``'s ``cpu_baseline`` leg import this package, and only to check the
+device results against or as the CPU baseline.  Nothing in ``numax_amd``
+imports it.
 
 Everything takes and returns row-major NumPy float64 arrays; the inputs are
 copied, never modified.

@@ -2,7 +2,7 @@
  * nmx_oracle.c -- CPU restatement of numax's float64 host algorithms for the
  * dense factor-and-solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker: tests/,
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and
  * nothing in the numax_amd product links, loads or calls it.
  *
