@@ -259,11 +259,11 @@ __global__ void __launch_bounds__(256) getf2_nopiv_kernel(float* __restrict__ A,
 // never raises it): rows strided over the workgroups, columns over the
 // threads (coalesced), one atomic per workgroup.
 constexpr int kAbsmaxBlocks = 256;
-__global__ void __launch_bounds__(256) absmax_kernel(int rows, int cols, const float* __restrict__ X, int ldx,
-                                                     unsigned* __restrict__ out) {
+__device__ __forceinline__ void absmax_rows(int bid, int nblk, int rows, int cols, const float* __restrict__ X,
+                                            int ldx, unsigned* __restrict__ out) {
   __shared__ float part[4];
   float m = 0.f;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+  for (int r = bid; r < rows; r += nblk) {
     const float* row = X + (int64_t)r * ldx;
     for (int c = threadIdx.x; c < cols; c += 256) {
       const float v = fabsf(row[c]);
@@ -277,6 +277,18 @@ __global__ void __launch_bounds__(256) absmax_kernel(int rows, int cols, const f
     m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
     atomicMax(out, __float_as_uint(m));
   }
+}
+
+// both operands of an update step in one launch: workgroups [0, ga) take X,
+// the rest Y
+__global__ void __launch_bounds__(256) absmax2_kernel(int ga, int rows, int cols, const float* __restrict__ X, int ldx,
+                                                      unsigned* __restrict__ out, int rows2, int cols2,
+                                                      const float* __restrict__ Y, int ldy,
+                                                      unsigned* __restrict__ out2) {
+  if ((int)blockIdx.x < ga)
+    absmax_rows(blockIdx.x, ga, rows, cols, X, ldx, out);
+  else
+    absmax_rows(blockIdx.x - ga, gridDim.x - ga, rows2, cols2, Y, ldy, out2);
 }
 
 __device__ __forceinline__ float pow2_scale(unsigned maxbits) {
@@ -305,18 +317,31 @@ struct Lo<1> {
 // dst[r][c] = scaled src[r][c] (transpose = 0) or scaled src[c][r] (transpose = 1);
 // dst is rows x cols row-major with ld = cols.
 template <int BF, int SPLIT>
-__global__ void convert_kernel(int rows, int cols, const float* __restrict__ src, int lds, int transpose,
-                               const unsigned* __restrict__ maxbits, typename Lo<BF>::t* __restrict__ hi,
-                               typename Lo<BF>::t* __restrict__ lo) {
+__device__ __forceinline__ void convert_part(int bid, int nblk, int rows, int cols, const float* __restrict__ src,
+                                             int lds, int transpose, const unsigned* __restrict__ maxbits,
+                                             typename Lo<BF>::t* __restrict__ hi, typename Lo<BF>::t* __restrict__ lo) {
   const float s = pow2_scale(*maxbits);
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
-       idx += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t idx = (int64_t)bid * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
+       idx += (int64_t)nblk * blockDim.x) {
     int r = (int)(idx / cols), c = (int)(idx % cols);
     float v = (transpose ? src[(int64_t)c * lds + r] : src[(int64_t)r * lds + c]) * s;
     typename Lo<BF>::t h = Lo<BF>::cvt(v);
     hi[idx] = h;
     if (SPLIT) lo[idx] = Lo<BF>::cvt((v - Lo<BF>::back(h)) * 2048.0f);
   }
+}
+
+// A (rows x cols, as is) on workgroups [0, ga), B (transposed) on the rest
+template <int BF, int SPLIT>
+__global__ void convert2_kernel(int ga, int rows, int cols, const float* __restrict__ A, int lda,
+                                const unsigned* __restrict__ maxA, typename Lo<BF>::t* __restrict__ ahi,
+                                typename Lo<BF>::t* __restrict__ alo, const float* __restrict__ B, int ldb,
+                                const unsigned* __restrict__ maxB, typename Lo<BF>::t* __restrict__ bhi,
+                                typename Lo<BF>::t* __restrict__ blo) {
+  if ((int)blockIdx.x < ga)
+    convert_part<BF, SPLIT>(blockIdx.x, ga, rows, cols, A, lda, 0, maxA, ahi, alo);
+  else
+    convert_part<BF, SPLIT>(blockIdx.x - ga, gridDim.x - ga, rows, cols, B, ldb, 1, maxB, bhi, blo);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,10 +948,11 @@ static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   lt* Bl = base + 3 * op;
   zero_u32k<<<1, 64, 0, ctx->stream>>>(f.maxbits, 2);
   const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
-  absmax_kernel<<<std::min(m, kAbsmaxBlocks), 256, 0, ctx->stream>>>(m, nb, L21, n, f.maxbits);
-  absmax_kernel<<<std::min(nb, kAbsmaxBlocks), 256, 0, ctx->stream>>>(nb, m, U12, n, f.maxbits + 1);
-  convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, L21, n, 0, f.maxbits, Ah, Al);
-  convert_kernel<BF, SPLIT><<<g1, 256, 0, ctx->stream>>>(m, nb, U12, n, 1, f.maxbits + 1, Bh, Bl);
+  // both scale words in one launch, then both conversions in one
+  const int ga = std::min(m, kAbsmaxBlocks), gb = std::min(nb, kAbsmaxBlocks);
+  absmax2_kernel<<<ga + gb, 256, 0, ctx->stream>>>(ga, m, nb, L21, n, f.maxbits, nb, m, U12, n, f.maxbits + 1);
+  convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.maxbits, Ah, Al, U12, n,
+                                                               f.maxbits + 1, Bh, Bl);
   NMX_HIP(ctx, hipGetLastError());
   const int tiles = ceil_div(m, MT);
   const int grid = ceil_div(tiles * tiles, kXcds) * kXcds;
