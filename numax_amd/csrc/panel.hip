@@ -113,10 +113,14 @@ struct PivKey<double> {
 // (parts > 1, grid 8 * parts of which every eighth takes part, so all
 // participants share one XCD and its L2): each part publishes its winner,
 // its key and (part 0) the diagonal row as epoch-tagged 64-bit words, and
-// every thread of every part spins on the words it stages -- one round trip,
-// no fences.  Slots alternate by epoch parity.
+// every thread of every part spins on the words it stages, no fences: all of
+// them in one round trip up to 16 parts; above that the keys and rows first,
+// then the winner's row and the diagonal row.  Slots alternate by epoch
+// parity: a part publishes column j+2 only after reading every part's key
+// for column j+1, which each part publishes only after its reads of column j.
 // ---------------------------------------------------------------------------
 constexpr int kLeafT = 512;  // threads per workgroup
+constexpr int kTwoPhaseParts = 16;  // above this many parts the exchange reads the winner's row only
 constexpr int kLeafWaves = kLeafT / 64;
 
 // ROWS rows per thread: 2 for a panel one workgroup holds (up to 1024
@@ -246,11 +250,14 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
       } else if (p == 0 && t >= 128 && t < 128 + W * VW) {
         mb_put(mbp + (size_t)kMaxParts * STRIDE + (t - 128), ((const uint32_t*)diag_v[par])[t - 128], epoch);
       }
-      // 5. stage every part's words (and the diagonal row), each validated by its epoch
-      const int words = parts * STRIDE + W * VW;
+      // 5. stage the parts' words, each validated by its epoch.  With many
+      // parts every workgroup reading every part's whole row crowds the few
+      // L2 lines they sit in, so it reads the keys and rows first, then only
+      // the winner's values and the diagonal row (two round trips, a tenth of
+      // the data; 10% faster over a 32-part leaf, 2% slower over 8 parts).
       uint32_t* stg = stage[par];
-      for (int idx = t; idx < words; idx += kLeafT) {
-        const uint64_t* src = idx < parts * STRIDE ? mbp + idx : mbp + (size_t)kMaxParts * STRIDE + (idx - parts * STRIDE);
+      const bool two_phase = parts > kTwoPhaseParts;  // uniform
+      auto stage_word = [&](const uint64_t* src, int dst) {
         uint64_t wv;
         long spins = 0;
         for (;;) {
@@ -259,14 +266,28 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
           if (aborted || ++spins > kSpinLimit) { aborted = 1; break; }
           __builtin_amdgcn_s_sleep(1);
         }
-        int dst;
-        if (idx < parts * STRIDE) {
-          const int q = idx / STRIDE, f = idx % STRIDE;  // mailbox slot: [row | key | values]
-          dst = q * PSTRIDE + (f == 0 ? W * VW + VW : (f <= VW ? W * VW + f - 1 : f - 1 - VW));
-        } else {
-          dst = kMaxParts * PSTRIDE + (idx - parts * STRIDE);
-        }
         stg[dst] = (uint32_t)(wv >> 32);
+      };
+      if (two_phase) {
+        constexpr int HW = VW + 1;  // header words of a slot: row, key
+        for (int idx = t; idx < parts * HW; idx += kLeafT) {
+          const int q = idx / HW, f = idx % HW;
+          stage_word(mbp + (size_t)q * STRIDE + f, q * PSTRIDE + (f == 0 ? W * VW + VW : W * VW + f - 1));
+        }
+      } else {
+        const int words = parts * STRIDE + W * VW;
+        for (int idx = t; idx < words; idx += kLeafT) {
+          const uint64_t* src =
+              idx < parts * STRIDE ? mbp + idx : mbp + (size_t)kMaxParts * STRIDE + (idx - parts * STRIDE);
+          int dst;
+          if (idx < parts * STRIDE) {
+            const int q = idx / STRIDE, f = idx % STRIDE;  // mailbox slot: [row | key | values]
+            dst = q * PSTRIDE + (f == 0 ? W * VW + VW : (f <= VW ? W * VW + f - 1 : f - 1 - VW));
+          } else {
+            dst = kMaxParts * PSTRIDE + (idx - parts * STRIDE);
+          }
+          stage_word(src, dst);
+        }
       }
       __syncthreads();
       // 6. the global winner: largest key, then smallest row (lane q reads
@@ -283,6 +304,15 @@ __global__ void __launch_bounds__(kLeafT) getf2_leaf(T* __restrict__ A, int lda,
       if (bk != 0) {
         brow = (int)wave_umin(pk == bk ? (uint32_t)pr : 0xFFFFFFFFu);
         bp = __ffsll((long long)__ballot(pk == bk && pr == brow)) - 1;
+      }
+      if (two_phase) {
+        // the winner's row values and the diagonal row (part bp published its
+        // values whether or not it had a candidate)
+        for (int idx = t; idx < 2 * W * VW; idx += kLeafT)
+          stage_word(idx < W * VW ? mbp + (size_t)bp * STRIDE + 1 + VW + idx
+                                  : mbp + (size_t)kMaxParts * STRIDE + (idx - W * VW),
+                     idx < W * VW ? bp * PSTRIDE + idx : kMaxParts * PSTRIDE + (idx - W * VW));
+        __syncthreads();
       }
       gk = bk;
       grow = brow;
