@@ -232,6 +232,17 @@ def test_lu_c2_scale_pivots_and_backward_error():
     _lu_parity(a, b, block=256, base=32)
 
 
+def test_lu_many_part_panels_match_oracle():
+    """n = 9216: the panels of the first 1024 columns span 18 workgroups of
+    512 rows, where the pivot exchange stages every part's key first and then
+    only the winner's row; pivots identical to the oracle's."""
+    oracle.set_threads(16)
+    n = 9216
+    a = R.device_normal(5, (n, n)).cpu().numpy()
+    b = R.device_normal(6, (n, 4)).cpu().numpy()
+    _lu_parity(a, b, block=256, base=32)
+
+
 @pytest.mark.parametrize("n", [1, 5, 64, 65, 300, 1000, 1300])
 def test_cholesky_matches_oracle(n):
     a = R.spd_matrix(300 + n, n)
