@@ -41,9 +41,9 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
 # n = 16384 for the LU paths, the 20,000-system batch for the batched leg.
 # Keyed by (kernel class, n or batch count); another size reports null.
 MEASURED_TRAFFIC = {
-    ("getf2_leaf", 16384): 13.60 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
-    ("mp_gemm", 16384): 823.25 * 2 ** 20,
-    ("batched_solve", 20000): 8 * 4460.04 * 2 ** 20,  # the eight class launches of one call
+    ("getf2_leaf", 16384): 13.59 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
+    ("mp_gemm", 16384): 823.45 * 2 ** 20,
+    ("batched_solve", 20000): 8 * 4462.17 * 2 ** 20,  # the eight class launches of one call
 }
 
 
