@@ -229,8 +229,9 @@ __device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k
   int myrow = lane < nb ? k + lane : -1;
   int mine = myrow;
   int nr = nb;
+  const int qv = lane < nb ? S.ipiv[k + lane] : 0;  // the pivots once, one per lane
   for (int j = 0; j < nb; j++) {
-    const int qrow = S.ipiv[k + j];
+    const int qrow = __builtin_amdgcn_readlane(qv, j);
     const unsigned long long hit = __ballot(lane < nr && myrow == qrow);
     int pos;
     if (hit) {

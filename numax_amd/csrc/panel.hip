@@ -410,8 +410,11 @@ __global__ void __launch_bounds__(256) laswp_gather_kernel(T* __restrict__ A, in
     int myrow = (t < w) ? k1 + t : -1;
     int mine = myrow;
     int nr = w;
+    // the pivots once, one per lane (a load inside the walk waited a scalar
+    // round trip per step)
+    const int qv = t < w ? ipiv[k1 + t] : 0;
     for (int j = 0; j < w; j++) {
-      const int q = ipiv[k1 + j];
+      const int q = __builtin_amdgcn_readlane(qv, j);
       const unsigned long long hit = __ballot(t < nr && myrow == q);
       int pos;
       if (hit) {
