@@ -41,7 +41,13 @@ typedef struct nmx_ctx nmx_ctx;
  * thread; calls on one context are not thread-safe.
  * Replaces numax's DeviceContext plumbing (numax/linalg/panel.mojo:10-24). */
 int nmx_ctx_create(int device, void* stream, nmx_ctx** out);
+/* While graphs captured on the context exist, destroying it only marks it:
+ * its buffers and stream stay valid for their replays and are released by
+ * the last nmx_graph_destroy. */
 int nmx_ctx_destroy(nmx_ctx* ctx);
+/* Points the context at another stream (NULL: a new stream it owns).  Work
+ * the new stream runs after this call waits for the work already queued on
+ * the old one (an event), since both use the context's buffers. */
 int nmx_ctx_set_stream(nmx_ctx* ctx, void* stream);
 int nmx_sync(nmx_ctx* ctx);
 /* Sticky device-side error word of the context (NMX_ERR_SYNC_TIMEOUT after a
@@ -63,7 +69,9 @@ const char* nmx_kernel_name(int kind);
  * in-place P A = L U, unit L below the diagonal, first-maximum pivots, full
  * row interchanges, an exact-zero pivot divided as 1e-30 and recorded in
  * `info` (not an error).  `block` is the panel width, `base` the recursive
- * panel's leaf width (0 picks the defaults 256 / 32). */
+ * panel's leaf width (0 picks the defaults 256 / 32).  n is at most 32768
+ * (the rows the cooperative pivoting leaf holds); a larger n returns
+ * NMX_ERR_UNSUPPORTED before any work is queued. */
 int nmx_getrf_f64(nmx_ctx* ctx, int n, double* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
 int nmx_getrf_f32(nmx_ctx* ctx, int n, float* A, int lda, int32_t* ipiv, int32_t* info, int block, int base);
 
@@ -177,7 +185,12 @@ int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, int preci
  * Per problem: info (first zero / non-positive pivot, 1-based; -1 for a
  * problem outside n 1..256 or kind 0/1, which is skipped), iters (refinement
  * steps) and berr (final backward error).  A problem whose matrix or
- * vectors would fall outside a_elems / b_elems is skipped the same way. */
+ * vectors would fall outside a_elems / b_elems is skipped the same way.
+ * Problems may share a matrix (equal or overlapping offA): each is factored
+ * in its own FP32 scratch, taken in index order from a pool of a_elems
+ * elements, and a problem whose scratch would not fit the pool (possible
+ * only when matrices overlap) is skipped with info -1.  Solutions are
+ * written at offB, so problems that share b also share x. */
 typedef struct nmx_batch_cfg {
   int max_iter;         /* refinement steps at most (default 10) */
   double tol;           /* target backward error (default 1e-15) */
@@ -250,8 +263,11 @@ int nmx_gels_f64(nmx_ctx* ctx, int m, int n, int nrhs, double* A, int lda, doubl
  * would need more workspace than an earlier call left (run it once first),
  * the pivoting LU drivers (their cooperative panels take host-side epochs)
  * and the mixed-precision drivers (their loops read norms on the host)
- * return NMX_ERR_UNSUPPORTED while capturing.  No reference counterpart:
- * for callers that replay the same batch shape (serving loops). */
+ * return NMX_ERR_UNSUPPORTED while capturing.  A live graph keeps every
+ * context buffer it reads: a workspace that grows afterwards gets a new
+ * buffer and the old one is freed with the context's last graph.  No
+ * reference counterpart: for callers that replay the same batch shape
+ * (serving loops). */
 typedef struct nmx_graph nmx_graph;
 int nmx_graph_begin(nmx_ctx* ctx);
 int nmx_graph_end(nmx_ctx* ctx, nmx_graph** out);

@@ -80,7 +80,8 @@ struct BatchArgs {
   int32_t* info;
   int32_t* iters;
   double* berr;
-  float* F;              // FP32 working copies, same offsets as A
+  float* F;              // FP32 working copies
+  const int64_t* offF;   // problem p's copy at F + offF[p] (a prefix sum of n^2)
   const int32_t* order;  // problem indices, by class, largest n first
   const int32_t* cls;    // [0..8) class begin, [8..16) class count
   int32_t* head;         // per-class queue heads
@@ -956,7 +957,7 @@ __global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs arg
     const int n = args.n[p];
     const double* A = args.A + args.offA[p];
     const double* b = args.B + args.offB[p];
-    float* F = args.F + args.offA[p];
+    float* F = args.F + args.offF[p];
 
     // FP32 working copy (a Cholesky problem keeps its lower triangle);
     // 32 loads in flight per lane, then the stores
@@ -1061,16 +1062,55 @@ __device__ __forceinline__ bool batch_ok(int nn, int kd, int64_t oa, int64_t ob,
          ob + nn <= b_elems;
 }
 
+// Offsets of the FP32 working copies: an exclusive prefix sum of n^2 over
+// the problems in index order, so problems that share (or overlap in) A
+// still factor into separate scratch.  One workgroup; a problem whose copy
+// would pass f_elems (only possible when problems overlap in A) gets -1 and
+// is skipped like an invalid one.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads) batch_offf_kernel(int count, const int32_t* __restrict__ n,
+                                                                  const uint8_t* __restrict__ kind,
+                                                                  const int64_t* __restrict__ offA,
+                                                                  const int64_t* __restrict__ offB, int64_t a_elems,
+                                                                  int64_t b_elems, int64_t f_elems,
+                                                                  int64_t* __restrict__ offF) {
+  __shared__ int64_t part[kScanThreads];
+  const int t = threadIdx.x;
+  const int chunk = (count + kScanThreads - 1) / kScanThreads;
+  const int lo = min(count, t * chunk), hi = min(count, lo + chunk);
+  int64_t sum = 0;
+  for (int i = lo; i < hi; i++) {
+    const int nn = n[i];
+    if (batch_ok(nn, kind[i], offA[i], offB[i], a_elems, b_elems)) sum += (int64_t)nn * nn;
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {  // Hillis-Steele inclusive scan
+    const int64_t v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;
+  for (int i = lo; i < hi; i++) {
+    const int nn = n[i];
+    if (batch_ok(nn, kind[i], offA[i], offB[i], a_elems, b_elems)) {
+      const int64_t need = (int64_t)nn * nn;
+      offF[i] = run + need <= f_elems ? run : -1;
+      run += need;
+    } else {
+      offF[i] = -1;
+    }
+  }
+}
+
 __global__ void batch_hist_kernel(int count, const int32_t* __restrict__ n, const uint8_t* __restrict__ kind,
-                                  const int64_t* __restrict__ offA, const int64_t* __restrict__ offB, int64_t a_elems,
-                                  int64_t b_elems, int32_t* __restrict__ hist, int32_t* __restrict__ info,
-                                  double* __restrict__ berr, int32_t* __restrict__ iters) {
+                                  const int64_t* __restrict__ offF, int32_t* __restrict__ hist,
+                                  int32_t* __restrict__ info, double* __restrict__ berr, int32_t* __restrict__ iters) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const int nn = n[i];
-  const bool ok = batch_ok(nn, kind[i], offA[i], offB[i], a_elems, b_elems);
-  if (ok) {
-    atomicAdd(&hist[kind[i] * (kBatchMaxN + 1) + nn], 1);
+  if (offF[i] >= 0) {
+    atomicAdd(&hist[kind[i] * (kBatchMaxN + 1) + n[i]], 1);
   } else {
     info[i] = -1;  // not a problem this kernel takes: reported, not solved
     iters[i] = 0;
@@ -1102,14 +1142,11 @@ __global__ void batch_scan_kernel(const int32_t* __restrict__ hist, int32_t* __r
 }
 
 __global__ void batch_scatter_kernel(int count, const int32_t* __restrict__ n, const uint8_t* __restrict__ kind,
-                                     const int64_t* __restrict__ offA, const int64_t* __restrict__ offB,
-                                     int64_t a_elems, int64_t b_elems, int32_t* __restrict__ cursor,
+                                     const int64_t* __restrict__ offF, int32_t* __restrict__ cursor,
                                      int32_t* __restrict__ order) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const int nn = n[i];
-  if (batch_ok(nn, kind[i], offA[i], offB[i], a_elems, b_elems))
-    order[atomicAdd(&cursor[kind[i] * (kBatchMaxN + 1) + nn], 1)] = i;
+  if (offF[i] >= 0) order[atomicAdd(&cursor[kind[i] * (kBatchMaxN + 1) + n[i]], 1)] = i;
 }
 
 __global__ void zero_i32_kernel(int32_t* p, int count) {
@@ -1160,13 +1197,16 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   for (int c = 0; c < 3; c++)
     if (cfg.teams_per_cu[c] <= 0 || cfg.teams_per_cu[c] > 64) return -15;
 
-  // workspace: FP32 working copies, then the queue
+  // workspace: FP32 working copies (as many elements as A), their offsets,
+  // then the queue
   const size_t f_bytes = ((sizeof(float) * (size_t)a_elems + 255) / 256) * 256;
+  const size_t o_bytes = ((sizeof(int64_t) * (size_t)count + 255) / 256) * 256;
   const size_t q_ints = 2 * (size_t)kBins + 2 * kClasses + kClasses + (size_t)count;
   void* ws = nullptr;
-  NMX_TRY(nmx_workspace(ctx, f_bytes + sizeof(int32_t) * q_ints, &ws));
+  NMX_TRY(nmx_workspace(ctx, f_bytes + o_bytes + sizeof(int32_t) * q_ints, &ws));
   float* F = (float*)ws;
-  int32_t* hist = (int32_t*)((char*)ws + f_bytes);
+  int64_t* offF = (int64_t*)((char*)ws + f_bytes);
+  int32_t* hist = (int32_t*)((char*)ws + f_bytes + o_bytes);
   int32_t* cursor = hist + kBins;
   int32_t* cls = cursor + kBins;
   int32_t* head = cls + 2 * kClasses;
@@ -1175,14 +1215,14 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   {
     ProfScope ps(ctx, K_OTHER, 0.0, 0.0);
     zero_i32_kernel<<<ceil_div(kBins, 256), 256, 0, ctx->stream>>>(hist, kBins);
-    batch_hist_kernel<<<ceil_div(count, 256), 256, 0, ctx->stream>>>(count, n, kind, offA, offB, a_elems, b_elems,
-                                                                       hist, info, berr, iters);
+    batch_offf_kernel<<<1, kScanThreads, 0, ctx->stream>>>(count, n, kind, offA, offB, a_elems, b_elems, a_elems,
+                                                           offF);
+    batch_hist_kernel<<<ceil_div(count, 256), 256, 0, ctx->stream>>>(count, n, kind, offF, hist, info, berr, iters);
     batch_scan_kernel<<<1, 64, 0, ctx->stream>>>(hist, cursor, cls, head);
-    batch_scatter_kernel<<<ceil_div(count, 256), 256, 0, ctx->stream>>>(count, n, kind, offA, offB, a_elems,
-                                                                          b_elems, cursor, order);
+    batch_scatter_kernel<<<ceil_div(count, 256), 256, 0, ctx->stream>>>(count, n, kind, offF, cursor, order);
     NMX_HIP(ctx, hipGetLastError());
   }
-  BatchArgs a{count, n, offA, offB, kind, A, B, X, info, iters, berr, F, order, cls, head, cfg.max_iter, cfg.tol};
+  BatchArgs a{count, n, offA, offB, kind, A, B, X, info, iters, berr, F, offF, order, cls, head, cfg.max_iter, cfg.tol};
   {
     ProfScope ps(ctx, K_BATCH, 0.0, 8.0 * (double)a_elems);
     // largest problems first; LU then Cholesky within a size class

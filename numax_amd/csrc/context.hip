@@ -75,11 +75,14 @@ NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
   return NMX_OK;
 }
 
-NMX_API int nmx_ctx_destroy(nmx_ctx* c) {
-  if (!c) return -1;
+// Releases every buffer of the context (the caller made sure no captured
+// graph can still replay kernels that read them).
+static void ctx_release(nmx_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto& r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
   for (auto e : c->pool) hipEventDestroy(e);
+  for (void* p : c->retired) hipFree(p);
+  if (c->switch_ev) hipEventDestroy(c->switch_ev);
   if (c->ws) hipFree(c->ws);
   if (c->mailbox) hipFree(c->mailbox);
   if (c->err) hipFree(c->err);
@@ -87,22 +90,43 @@ NMX_API int nmx_ctx_destroy(nmx_ctx* c) {
   if (c->mp_cache && c->mp_cache_free) c->mp_cache_free(c->mp_cache);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;
+}
+
+NMX_API int nmx_ctx_destroy(nmx_ctx* c) {
+  if (!c) return -1;
+  if (c->destroy_pending) return -1;
+  if (c->live_graphs > 0) {
+    // graphs captured on this context still hold its buffers and stream:
+    // the last nmx_graph_destroy releases them
+    c->destroy_pending = true;
+    return NMX_OK;
+  }
+  ctx_release(c);
   return NMX_OK;
 }
 
 NMX_API int nmx_ctx_set_stream(nmx_ctx* c, void* stream) {
   if (!c) return -1;
+  if (c->capturing) return NMX_ERR_UNSUPPORTED;
+  hipStream_t next = (hipStream_t)stream;
+  if (next && next == c->stream) return NMX_OK;
+  if (c->own_stream && c->live_graphs > 0) return NMX_ERR_UNSUPPORTED;  // the graphs replay on it
+  if (!next) {
+    NMX_HIP(c, hipStreamCreate(&next));
+  }
+  // work already queued on the old stream (the shared workspace, mailbox and
+  // caches it uses) completes before anything the new stream runs next
+  if (c->stream) {
+    if (!c->switch_ev) NMX_HIP(c, hipEventCreateWithFlags(&c->switch_ev, hipEventDisableTiming));
+    NMX_HIP(c, hipEventRecord(c->switch_ev, c->stream));
+    NMX_HIP(c, hipStreamWaitEvent(next, c->switch_ev, 0));
+  }
   if (c->own_stream && c->stream) {
     hipStreamSynchronize(c->stream);
     hipStreamDestroy(c->stream);
-    c->own_stream = false;
   }
-  if (stream) {
-    c->stream = (hipStream_t)stream;
-  } else {
-    NMX_HIP(c, hipStreamCreate(&c->stream));
-    c->own_stream = true;
-  }
+  c->stream = next;
+  c->own_stream = stream == nullptr;
   return NMX_OK;
 }
 
@@ -133,7 +157,11 @@ int nmx_workspace(nmx_ctx* c, size_t bytes, void** out) {
   if (bytes > c->ws_bytes) {
     if (c->capturing) return NMX_ERR_UNSUPPORTED;  // run the call once before capturing it
     NMX_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->ws) hipFree(c->ws);
+    if (c->ws) {
+      // a live graph may replay kernels that read the old buffer
+      if (c->live_graphs > 0) c->retired.push_back(c->ws);
+      else hipFree(c->ws);
+    }
     c->ws = nullptr;
     c->ws_bytes = 0;
     if (hipMalloc(&c->ws, bytes) != hipSuccess) return NMX_ERR_ALLOC;
@@ -298,6 +326,7 @@ struct nmx_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
+  nmx_ctx* ctx = nullptr;  // whose buffers the replayed kernels read
 };
 
 NMX_API int nmx_graph_begin(nmx_ctx* c) {
@@ -325,6 +354,8 @@ NMX_API int nmx_graph_end(nmx_ctx* c, nmx_graph** out) {
   r->graph = g;
   r->exec = e;
   r->stream = c->stream;
+  r->ctx = c;
+  c->live_graphs++;
   *out = r;
   return NMX_OK;
 }
@@ -336,8 +367,18 @@ NMX_API int nmx_graph_launch(nmx_graph* g) {
 
 NMX_API int nmx_graph_destroy(nmx_graph* g) {
   if (!g) return NMX_OK;
+  if (g->stream) hipStreamSynchronize(g->stream);  // no replay still running
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
+  nmx_ctx* c = g->ctx;
   delete g;
+  if (c && --c->live_graphs == 0) {
+    if (c->destroy_pending) {
+      ctx_release(c);
+    } else {
+      for (void* p : c->retired) hipFree(p);
+      c->retired.clear();
+    }
+  }
   return NMX_OK;
 }

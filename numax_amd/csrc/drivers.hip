@@ -118,6 +118,7 @@ static int getrf_api(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t*
   if (!info) return -6;
   if (block < 0) return -7;
   if (base < 0) return -8;
+  if (n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;  // the pivoting leaf's row capacity
   if (n == 0) return zero_info(ctx, info);
   return getrf_impl<T>(ctx, n, A, lda, ipiv, info, block, base);
 }

@@ -1356,6 +1356,7 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   if (cfg.solver < 0 || cfg.solver > 2) return -7;
   if (cfg.solver != 1 && (cfg.gmres_restart < 1 || cfg.gmres_restart > 100)) return -7;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
+  if (cfg.pivot && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;  // checked before anything is allocated
   nmx_ir_report local{};
   nmx_ir_report* R = rep ? rep : &local;
   *R = nmx_ir_report{};
@@ -1477,6 +1478,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   if (cfg.block <= 0) cfg.block = 256;
   if (cfg.block % 64) return -10;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
+  if (cfg.pivot && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;
   std::vector<nmx_ir_report> local;
   if (!reps) {
     local.resize(nrhs);

@@ -32,6 +32,9 @@ constexpr int kLeafW = 32;           // columns one cooperative leaf factors
 constexpr int kLeafThreads = 1024;   // threads per participating workgroup
 constexpr int kMaxParts = 32;        // participating workgroups at most
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
+// Rows the cooperative pivoting leaf holds: kMaxParts workgroups of 512
+// threads with two rows each.  lu_factor / getrf take n up to this.
+constexpr int kLeafMaxRows = 32768;
 
 // Kernel classes timed by the context's optional event profiler.
 enum NmxKernel {
@@ -67,6 +70,15 @@ struct nmx_ctx {
   // grow-only scratch
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // captured graphs still alive that may replay kernels reading the
+  // context's buffers: while any exists, a workspace that grows keeps its
+  // old buffer (retired, freed with the last graph), and nmx_ctx_destroy
+  // defers the release of every buffer to the last nmx_graph_destroy
+  int live_graphs = 0;
+  bool destroy_pending = false;
+  std::vector<void*> retired;
+  // orders a newly set stream after the work queued on the previous one
+  hipEvent_t switch_ev = nullptr;
   // cross-workgroup mailbox for the cooperative panel: flags then slots
   void* mailbox = nullptr;
   size_t mailbox_bytes = 0;

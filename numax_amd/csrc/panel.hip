@@ -999,14 +999,14 @@ int getf2_leaf_launch(nmx_ctx* ctx, int n, T* A, int lda, int r0, int c0, int w,
   // of 512 rows), two otherwise
   const bool one_row = m > kLeafT * 2 && m <= kLeafT * kMaxParts;
   int parts = ceil_div(m, one_row ? kLeafT : kLeafT * 2);
-  if (parts > kMaxParts) return NMX_ERR_UNSUPPORTED;  // n - r0 <= 32768 rows
+  if (parts > kMaxParts) return NMX_ERR_UNSUPPORTED;  // n - r0 <= kLeafMaxRows
   uint32_t epoch0 = 0;
   NMX_TRY(nmx_epoch_reserve(ctx, (uint32_t)w, &epoch0));
   uint64_t* mb = (uint64_t*)ctx->mailbox;
   int* err = ctx->err;
   {
     ProfScope ps(ctx, K_LEAF, (double)m * w * w, 2.0 * sizeof(T) * (double)m * w);
-    // At most 8 x 32 workgroups of 1024 threads: one per CU of an otherwise
+    // At most 8 x 32 workgroups of 512 threads: one per CU of an otherwise
     // idle device (the stream orders this launch after the previous kernel).
     // A participant that is not resident makes the others' bounded waits
     // expire, which the context reports as NMX_ERR_SYNC_TIMEOUT.

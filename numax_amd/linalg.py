@@ -147,12 +147,14 @@ class LU:
         return sign, total
 
 
-def lu_factor(a, block: int = 256, base: int = 32) -> LU:
+def lu_factor(a, block: int = 16, base: int = 16) -> LU:
     """Factor ``a`` into ``P L U`` (numax ``lu_factor``, lu.mojo:454-701).
 
     ``block`` is the panel width and ``base`` the recursive panel's leaf width
-    (the reference's parameters of the same names); a singular matrix is not
-    reported as an error (see ``LU.singular``).
+    (the reference's parameters of the same names, with its device defaults
+    16 / 16, lu.mojo:454-459); the pivots do not depend on them, the speed
+    does: 256 / 32 is the fastest shape on MI355X at large n.  A singular
+    matrix is not reported as an error (see ``LU.singular``).
     """
     work = _device_tensor(a).clone()
     n = _square(work, "lu_factor")
@@ -184,15 +186,17 @@ def laswp(b, pivots) -> torch.Tensor:
     return x
 
 
-def solve(a, b, block: int = 256) -> torch.Tensor:
+def solve(a, b, block: int = 16) -> torch.Tensor:
     """``lu_factor`` then the solve at the same device (basic.mojo:55-146)."""
     return lu_factor(a, block=block).solve(b)
 
 
-def cholesky(a, block: int = 256) -> torch.Tensor:
+def cholesky(a, block: int = 32) -> torch.Tensor:
     """Lower ``L`` with ``L L^T = a`` (cholesky.mojo:233-396).  Reads only the
     lower triangle; raises naming the 0-based index of the first non-positive
-    pivot, after the whole factorization has run."""
+    pivot, after the whole factorization has run.  ``block`` defaults to
+    numax's device default (32, cholesky.mojo:233-237); 256 is the fastest
+    panel width on MI355X at large n."""
     work = _device_tensor(a).clone()
     n = _square(work, "cholesky")
     info = torch.zeros(1, dtype=torch.int32, device=work.device)

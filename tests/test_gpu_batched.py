@@ -157,3 +157,83 @@ def test_batched_graph_replays_match_eager():
         np.testing.assert_array_equal(x_graph, eager.x.cpu().numpy())
         assert (info_graph == 0).all()
     g.close()
+
+
+def test_batched_graph_survives_workspace_growth():
+    """A live graph keeps the workspace it was captured with: calls that grow
+    the context's workspace between replays (a larger batch, an LU solve with
+    many right-hand sides) must not free it under the graph."""
+    import torch
+    sizes = [40, 150, 256, 9]
+    mats, rhss, kinds = _systems(sizes, seed=31)
+    a, b, n, oa, ob = nl.pack_systems(mats, rhss)
+    g = nl.batched_solve_graph(a, b, n, kinds, oa, ob)
+    rng = np.random.default_rng(8)
+    for rep in range(3):
+        # grow the workspace past anything used so far, each time by more
+        big = [int(s) for s in rng.integers(200, 257, size=60 * (rep + 1))]
+        bm, br, bk = _systems(big, seed=100 + rep)
+        ba, bb, bn, boa, bob = nl.pack_systems(bm, br)
+        other = nl.batched_solve(ba, bb, bn, bk, boa, bob)
+        lu = nl.lu_factor(rng.standard_normal((600, 600)), block=256, base=32)
+        lu.solve(rng.standard_normal((600, 300 * (rep + 1))))
+        torch.cuda.synchronize()
+        assert (other.info.cpu().numpy() == 0).all()
+        b2 = b + rng.standard_normal(b.shape)
+        g.b.copy_(torch.from_numpy(b2))
+        res = g.replay()
+        torch.cuda.synchronize()
+        eager = nl.batched_solve(a, b2, n, kinds, oa, ob)
+        np.testing.assert_array_equal(res.x.cpu().numpy(), eager.x.cpu().numpy())
+        assert (res.info.cpu().numpy() == 0).all()
+    g.close()
+
+
+def test_batched_two_streams_in_a_row():
+    """Two asynchronous calls under different torch streams share the
+    context's workspace: the second is ordered after the first."""
+    import torch
+    m1, r1, k1 = _systems([256, 200, 180, 256] * 8, seed=41)
+    m2, r2, k2 = _systems([250, 60, 256, 7] * 8, seed=42)
+    p1 = nl.pack_systems(m1, r1)
+    p2 = nl.pack_systems(m2, r2)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s1):
+        res1 = nl.batched_solve(p1[0], p1[1], p1[2], k1, p1[3], p1[4])
+    with torch.cuda.stream(s2):
+        res2 = nl.batched_solve(p2[0], p2[1], p2[2], k2, p2[3], p2[4])
+    torch.cuda.synchronize()
+    _check(m1, r1, k1, res1, res1.info.cpu().numpy(), res1.berr.cpu().numpy())
+    _check(m2, r2, k2, res2, res2.info.cpu().numpy(), res2.berr.cpu().numpy())
+
+
+def test_batched_problems_sharing_a_matrix():
+    """Problems that share one A (several right-hand sides) are factored in
+    separate scratch; with a pool too small for both, the later one is
+    skipped with info -1."""
+    rng = np.random.default_rng(12)
+    n0 = 150
+    a0 = rng.uniform(-1.0, 1.0, (n0, n0)) + 0.1 * n0 * np.eye(n0)
+    bs = [rng.standard_normal(n0) for _ in range(3)]
+    # the A buffer is padded so the scratch pool (as many elements as A) fits three copies
+    abuf = np.zeros(3 * n0 * n0)
+    abuf[:n0 * n0] = a0.ravel()
+    bbuf = np.concatenate(bs)
+    n = np.full(3, n0, dtype=np.int32)
+    oa = np.zeros(3, dtype=np.int64)
+    ob = np.arange(3, dtype=np.int64) * n0
+    kinds = np.zeros(3, dtype=np.uint8)
+    res = nl.batched_solve(abuf, bbuf, n, kinds, oa, ob)
+    info = res.info.cpu().numpy()
+    assert (info == 0).all()
+    for p in range(3):
+        x = res.solution(p).cpu().numpy()
+        assert oracle.backward_error(a0, x, bs[p]) <= 2e-15
+        xo = _oracle_x(a0, bs[p], 0)
+        assert np.max(np.abs(x - xo)) <= 1e-12 * max(1.0, np.max(np.abs(xo)))
+    # a pool of 2 n0^2 holds two copies: the third problem is skipped
+    res2 = nl.batched_solve(abuf[:2 * n0 * n0], bbuf, n, kinds, oa, ob)
+    info2 = res2.info.cpu().numpy()
+    assert info2[0] == 0 and info2[1] == 0 and info2[2] == -1
+    np.testing.assert_array_equal(res2.x.cpu().numpy()[:2 * n0], res.x.cpu().numpy()[:2 * n0])
