@@ -826,7 +826,7 @@ __global__ void vec_f32_add_f64(int n, const float* __restrict__ y, double* __re
 // ===========================================================================
 struct MpFactor {
   int n = 0, block = 256, prec = 2;  // prec: 0 fp16, 1 bf16, 2 fp16 split, 3 fp32
-  int pivot = 1;                     // 1: partial pivoting (GEPP); 0: butterflies, no pivoting
+  int pivot = 1;                     // 1: partial pivoting (GEPP); 2: tournament pivoting; 0: butterflies, no pivoting
   int32_t* ipiv = nullptr;
   int32_t* perm = nullptr;           // net row permutation of ipiv
   float* F = nullptr;                // n x n FP32 factors (of W^T A V without pivoting)
@@ -838,6 +838,8 @@ struct MpFactor {
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
   unsigned* maxbits = nullptr;       // [2] per step
   uint64_t* xpub = nullptr;          // TRSV published solution words (n)
+  void* ts = nullptr;                // tournament leaf scratch
+  size_t ts_bytes = 0;
   uint32_t epoch = 0;
   bool ok = false;
 };
@@ -845,7 +847,8 @@ struct MpFactor {
 // Device buffers of the factorization, cached in the context and grown on
 // demand so repeated solves do not allocate.
 struct MpBuffers {
-  size_t F_elems = 0, d_elems = 0, inv_elems = 0, piv_elems = 0, lo_bytes = 0, pub_elems = 0;
+  size_t F_elems = 0, d_elems = 0, inv_elems = 0, piv_elems = 0, lo_bytes = 0, pub_elems = 0, ts_bytes = 0;
+  void* ts = nullptr;
   float* F = nullptr;
   float* d = nullptr;
   float* inv = nullptr;
@@ -868,6 +871,7 @@ static void mp_buffers_free(void* p) {
   hipFree(b->lo);
   hipFree(b->maxbits);
   hipFree(b->xpub);
+  hipFree(b->ts);
   delete b;
 }
 
@@ -921,6 +925,15 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.maxbits = b->maxbits;
   f.xpub = b->xpub;
   f.epoch = b->epoch;
+  if (f.pivot == 2) {
+    const size_t want = tslu_scratch_bytes(n);
+    if (want > b->ts_bytes) {
+      NMX_TRY(grow(ctx, b->ts, b->ts_bytes, want, 1));
+      NMX_TRY(tslu_scratch_init(ctx, b->ts));
+    }
+    f.ts = b->ts;
+    f.ts_bytes = b->ts_bytes;
+  }
   return NMX_OK;
 }
 
@@ -998,6 +1011,12 @@ static int mp_update_hook(nmx_ctx* ctx, void* user, int k, int nb, int m) {
   }
 }
 
+static int tslu_hook(nmx_ctx* ctx, void* user, int n, float* A, int lda, int r0, int c0, int w, int32_t* ipiv,
+                     int32_t* info) {
+  MpFactor& f = *(MpFactor*)user;
+  return tslu_leaf_launch(ctx, f.ts, f.ts_bytes, n, A, lda, r0, c0, w, ipiv, info);
+}
+
 static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed, int stop_after_transform = 0) {
   const int n = f.n;
   if (f.pivot) {
@@ -1011,7 +1030,7 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
     }
     if (stop_after_transform) return zero_info(ctx, f.info);
     NMX_TRY(getrf_impl_ex<float>(ctx, n, f.F, n, f.ipiv, f.info, f.block, kLeafW,
-                                 f.prec == 3 ? nullptr : mp_update_hook, &f));
+                                 f.prec == 3 ? nullptr : mp_update_hook, &f, f.pivot == 2 ? tslu_hook : nullptr));
     NMX_TRY(perm_from_ipiv(ctx, n, f.ipiv, f.perm));
     const int nblk = ceil_div(n, TB);
     tri_inv_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL);
@@ -1356,7 +1375,8 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   if (cfg.solver < 0 || cfg.solver > 2) return -7;
   if (cfg.solver != 1 && (cfg.gmres_restart < 1 || cfg.gmres_restart > 100)) return -7;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
-  if (cfg.pivot && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;  // checked before anything is allocated
+  if (cfg.pivot < 0 || cfg.pivot > 2) return -7;
+  if (cfg.pivot == 1 && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;  // checked before anything is allocated
   nmx_ir_report local{};
   nmx_ir_report* R = rep ? rep : &local;
   *R = nmx_ir_report{};
@@ -1478,7 +1498,8 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   if (cfg.block <= 0) cfg.block = 256;
   if (cfg.block % 64) return -10;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
-  if (cfg.pivot && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;
+  if (cfg.pivot < 0 || cfg.pivot > 2) return -10;
+  if (cfg.pivot == 1 && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;
   std::vector<nmx_ir_report> local;
   if (!reps) {
     local.resize(nrhs);
@@ -1670,9 +1691,11 @@ NMX_API int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, i
   if (lda < n) return -4;
   if (precision < 0 || precision > 3) return -5;
   if (block <= 0 || block % 64) return -6;
+  if (pivot < 0 || pivot > 2) return -9;
+  if (pivot == 1 && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;
   MpFactor f;
-  int st = mp_alloc(ctx, f, n, block, precision);
   f.pivot = pivot;
+  int st = mp_alloc(ctx, f, n, block, precision);
   if (st == NMX_OK) st = mp_factor(ctx, f, A, lda, seed, stage == 0);
   if (st == NMX_OK) {
     hipMemcpyAsync(F, f.F, sizeof(float) * (size_t)n * n, hipMemcpyDeviceToDevice, ctx->stream);

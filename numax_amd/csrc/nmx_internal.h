@@ -158,9 +158,21 @@ int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info,
 // Block-level trailing update hook: A22 -= L21 U12 for the step at column k
 // with panel width nb and trailing order m.
 typedef int (*TrailingUpdate)(nmx_ctx* ctx, void* user, int k, int nb, int m);
+// Leaf hook of the recursive panel (FP32 only): factors columns c0..c0+w over
+// rows r0..n, writes ipiv[r0..r0+w) and replays the interchanges over the
+// other columns, like getf2_leaf_launch with swap_others.
+typedef int (*LeafHook)(nmx_ctx* ctx, void* user, int n, float* A, int lda, int r0, int c0, int w, int32_t* ipiv,
+                        int32_t* info);
 template <typename T>
 int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base,
-                  TrailingUpdate update, void* user);
+                  TrailingUpdate update, void* user, LeafHook leaf = nullptr);
+
+// Tournament-pivoted leaf of the mixed-precision factorization (tslu.hip):
+// scratch of tslu_scratch_bytes(n), cleared once by tslu_scratch_init.
+size_t tslu_scratch_bytes(int n);
+int tslu_scratch_init(nmx_ctx* ctx, void* scratch);
+int tslu_leaf_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int r0, int c0,
+                     int w, int32_t* ipiv, int32_t* info);
 
 template <typename T>
 int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);

@@ -29,6 +29,10 @@ __all__ = ["LU", "NumaxError", "lu_factor", "solve", "solve_mpir", "cholesky", "
            "gemm", "PRECISIONS"]
 
 # trailing-update precisions of the mixed-precision solver (nmx_ir_cfg.precision)
+# how the low-precision factor pivots: per-column partial pivoting (numax's
+# pivots), tournament pivoting (one reduction per 32-column leaf), or random
+# butterflies with no pivoting
+PIVOTS = {"partial": 1, "tournament": 2, "butterfly": 0}
 PRECISIONS = {"fp16": 0, "bf16": 1, "fp16s": 2, "fp32": 3}
 
 
@@ -402,9 +406,9 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
         block = autotune.cached("mpir", A.device.index or 0, n=n, precision=precision) or 256
     cfg.block = int(block)
     cfg.seed = seed
-    if pivot not in ("partial", "butterfly"):
-        raise NumaxError("solve_mpir: pivot must be 'partial' or 'butterfly'")
-    cfg.pivot = 1 if pivot == "partial" else 0
+    if pivot not in PIVOTS:
+        raise NumaxError(f"solve_mpir: pivot must be one of {sorted(PIVOTS)}")
+    cfg.pivot = PIVOTS[pivot]
     if solver not in SOLVERS:
         raise NumaxError(f"solve_mpir: solver must be one of {sorted(SOLVERS)}")
     cfg.solver = SOLVERS[solver]
