@@ -61,24 +61,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def haar(seed: int, n: int, torch):
-    from numax_amd import random as R
-    g = R.device_normal(seed, (n, n))
-    q, r = torch.linalg.qr(g)
-    d = torch.sign(torch.diagonal(r))
-    d[d == 0] = 1.0
-    del g, r
-    return q * d  # Mezzadri's sign fix: Haar-distributed
-
-
 def c3_matrix(n: int, kappa: float, torch, seed: int = 3):
-    u = haar(seed, n, torch)
-    v = haar(seed + 100, n, torch)
-    i = torch.arange(n, dtype=torch.float64, device="cuda")
-    sigma = kappa ** (-i / (n - 1))
-    a = (u * sigma) @ v.T
-    del u, v
-    return a.contiguous()
+    from numax_amd import random as R
+    return R.device_svd_matrix(n, kappa, seed=seed)
 
 
 def backward_error(a, x, b) -> float:
@@ -115,40 +100,6 @@ def cpu_baseline(n_sample: int, threads: int) -> dict:
             "gflops": rate / 1e9, "finite": bool(np.isfinite(x).all())}
 
 
-def c4_batch(count: int, seed: int, torch):
-    """20,000-style uneven batch: n ~ U{8..256}; even problems LU with
-    A ~ U[-1,1] + 0.1 n I, odd problems Cholesky with A = G G^T / n + I;
-    b ~ N(0,1).  Generated on the GPU size by size and packed back to back."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    n = rng.integers(8, 257, size=count).astype(np.int32)
-    kind = (np.arange(count) % 2).astype(np.uint8)
-    offa = np.zeros(count, dtype=np.int64)
-    offb = np.zeros(count, dtype=np.int64)
-    offa[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
-    offb[1:] = np.cumsum(n.astype(np.int64))[:-1]
-    a = torch.empty(int((n.astype(np.int64) ** 2).sum()), dtype=torch.float64, device="cuda")
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    for size in np.unique(n):
-        size = int(size)
-        for kd in (0, 1):
-            idx = np.flatnonzero((n == size) & (kind == kd))
-            if idx.size == 0:
-                continue
-            eye = torch.eye(size, dtype=torch.float64, device="cuda")
-            if kd == 0:
-                m = torch.rand((idx.size, size, size), dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
-                m += 0.1 * size * eye
-            else:
-                g = torch.randn((idx.size, size, size), dtype=torch.float64, device="cuda", generator=gen)
-                m = torch.bmm(g, g.transpose(1, 2).contiguous()) / size + eye
-            dst = torch.as_tensor(offa[idx], device="cuda")[:, None] + torch.arange(size * size, device="cuda")
-            a[dst.reshape(-1)] = m.reshape(-1)
-    b = torch.randn(int(n.sum()), dtype=torch.float64, device="cuda", generator=gen)
-    return a, b, n, kind, offa, offb
-
-
 def batched_leg(args, torch) -> dict:
     """Times the uneven batch (one nmx_batched_gesv per step) and reports its
     HBM roofline: algorithmic bytes = sum(8 n^2 + 16 n) (A and b read once,
@@ -159,7 +110,8 @@ def batched_leg(args, torch) -> dict:
 
     from numax_amd import _lib
     from numax_amd import linalg as nl
-    a, b, n, kind, offa, offb = c4_batch(args.batch_count, 4, torch)
+    from numax_amd import random as R
+    a, b, n, kind, offa, offb = R.device_uneven_batch(args.batch_count, 4)
     lib = _lib.load()
     ctx = nl._ctx(a)
     N = torch.as_tensor(n, device="cuda")

@@ -1,0 +1,180 @@
+"""The GPU workloads at their full sizes, each checked by quantities computed
+independently of this library's kernels (FP64 torch/rocBLAS products on the
+device, NumPy/LAPACK and the CPU oracle on sampled problems):
+
+* mixed-precision LU-IR at n = 16384 on A = U diag(sigma) V^T (Haar U, V),
+  kappa in {1e2, 1e4, 1e6}, one and 64 right-hand sides: the normwise
+  backward error of every column recomputed in FP64 <= 1e-15, and x within
+  kappa * 1e-13 (relative) of rocSOLVER's FP64 solve;
+* 20,000 uneven systems (n ~ U{8..256}, half LU, half Cholesky) in one call:
+  every problem's backward error recomputed in FP64 <= 2e-15, info 0, and a
+  seeded sample of 40 problems against the oracle's register-tier solves;
+* 65,536 symmetric eigenproblems (n in {24, 32, 48, 64}, Hamiltonians and
+  (G + G^T)/2): eigen-residual and orthogonality of every matrix, sorted
+  eigenvalues of a sample against LAPACK and the oracle;
+* 4096 least-squares problems 1024 x 32: the normal-equation residual
+  A^T (b - A x) of every problem, a sample against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from numax_amd import linalg as nl
+from numax_amd import random as R
+
+pytestmark = pytest.mark.gpu
+
+N3 = 16384
+
+
+def _eta(A, X, B):
+    """max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf), FP64 (torch)."""
+    X2 = X if X.dim() == 2 else X[:, None]
+    B2 = B if B.dim() == 2 else B[:, None]
+    Rm = B2 - A @ X2
+    anorm = A.abs().sum(1).max()
+    return float((Rm.abs().max(0).values / (anorm * X2.abs().max(0).values)).max())
+
+
+@pytest.fixture(scope="module")
+def haar_pair():
+    import torch
+    u = R.device_haar(3, N3)
+    v = R.device_haar(103, N3)
+    torch.cuda.synchronize()
+    yield u, v
+    del u, v
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+def test_c3_mpir_one_rhs(haar_pair, kappa):
+    import torch
+    A = R.device_svd_matrix(N3, kappa, factors=haar_pair)
+    b = R.device_normal(4, (N3,))
+    x, rep = nl.solve_mpir(A, b, precision="fp16s")
+    assert rep["converged"], rep
+    eta = _eta(A, x, b)
+    assert eta <= 1e-15, (kappa, eta, rep)
+    xr = torch.linalg.solve(A, b)
+    rel = float((x - xr).abs().max() / xr.abs().max())
+    assert rel <= kappa * 1e-13, (kappa, rel)
+
+
+def test_c3_mpir_64_rhs(haar_pair):
+    A = R.device_svd_matrix(N3, 1e6, factors=haar_pair)
+    B = R.device_normal(5, (N3, 64))
+    X, reps = nl.solve_mpir(A, B, precision="fp16s")
+    assert all(r["converged"] for r in reps)
+    assert _eta(A, X, B) <= 1e-15
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_c3_single_product_kappa_1e2(haar_pair, precision):
+    A = R.device_svd_matrix(N3, 1e2, factors=haar_pair)
+    b = R.device_normal(4, (N3,))
+    x, rep = nl.solve_mpir(A, b, precision=precision, max_iter=30)
+    assert rep["converged"], rep
+    assert _eta(A, x, b) <= 1e-15
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("kappa", [1e4, 1e6])
+def test_c3_single_product_reports_honestly(haar_pair, precision, kappa):
+    # a single FP16 / BF16 product per update: whatever the refinement
+    # reaches, the report's backward error is the one the FP64 check measures
+    A = R.device_svd_matrix(N3, kappa, factors=haar_pair)
+    b = R.device_normal(4, (N3,))
+    x, rep = nl.solve_mpir(A, b, precision=precision, max_iter=30, raise_on_failure=False)
+    eta = _eta(A, x, b)
+    if rep["converged"]:
+        assert rep["backward_error"] <= 1e-15 and eta <= 2e-15, (eta, rep)
+    else:
+        assert rep["backward_error"] > 1e-15
+        assert abs(eta - rep["backward_error"]) <= 1e-3 * eta, (eta, rep["backward_error"])
+
+
+def test_c4_uneven_batch_20000():
+    import torch
+    a, b, n, kind, oa, ob = R.device_uneven_batch(20000, 4)
+    res = nl.batched_solve(a, b, n, kind, oa, ob)
+    info = res.info.cpu().numpy()
+    berr = res.berr.cpu().numpy()
+    assert (info == 0).all()
+    assert berr.max() <= 1e-15
+    X = res.x
+    # every problem's backward error in FP64, size by size (batched products)
+    worst = 0.0
+    for size in np.unique(n):
+        size = int(size)
+        idx = np.flatnonzero(n == size)
+        ia = torch.as_tensor(oa[idx], device="cuda")[:, None] + torch.arange(size * size, device="cuda")
+        ib = torch.as_tensor(ob[idx], device="cuda")[:, None] + torch.arange(size, device="cuda")
+        Am = a[ia].reshape(-1, size, size)
+        xm = X[ib][:, :, None]
+        bm = b[ib][:, :, None]
+        r = (bm - torch.bmm(Am, xm)).abs().amax(dim=(1, 2))
+        eta = r / (Am.abs().sum(2).amax(1) * xm.abs().amax(dim=(1, 2)))
+        worst = max(worst, float(eta.max()))
+    assert worst <= 2e-15, worst
+    # a seeded sample against the oracle's register-tier algorithms
+    rng = np.random.default_rng(99)
+    for p in rng.choice(len(n), size=40, replace=False):
+        k = int(n[p])
+        ap = a[int(oa[p]):int(oa[p]) + k * k].reshape(k, k).cpu().numpy()
+        bp = b[int(ob[p]):int(ob[p]) + k].cpu().numpy()
+        xp = X[int(ob[p]):int(ob[p]) + k].cpu().numpy()
+        if kind[p] == 0:
+            lu, perm, _ = oracle.array_lu_factor(ap)
+            xo = oracle.array_lu_solve(lu, perm, bp)
+        else:
+            xo = oracle.array_cholesky_solve(oracle.array_cholesky(ap), bp)
+        assert np.max(np.abs(xp - xo)) <= 1e-12 * max(1.0, np.abs(xo).max()), (p, k)
+
+
+def test_c5_eigh_65536():
+    import torch
+    a, n, oa, ow = R.eigh_batch(65536, 5)
+    w, v = nl.eigh_batched(a, n, oa, ow)
+    A = torch.as_tensor(a, device="cuda")
+    rng = np.random.default_rng(7)
+    for size in (24, 32, 48, 64):
+        idx = np.flatnonzero(n == size)
+        assert idx.size == 16384
+        ia = torch.as_tensor(oa[idx], device="cuda")[:, None] + torch.arange(size * size, device="cuda")
+        iw = torch.as_tensor(ow[idx], device="cuda")[:, None] + torch.arange(size, device="cuda")
+        Am = A[ia].reshape(-1, size, size).double()
+        V = v[ia].reshape(-1, size, size).double()
+        W = w[iw].double()
+        scale = Am.abs().sum(2).amax(1)  # ||A||_inf >= ||A||_2
+        res = (torch.bmm(Am, V) - V * W[:, None, :]).abs().amax(dim=(1, 2)) / scale
+        orth = (torch.bmm(V.transpose(1, 2).contiguous(), V) - torch.eye(size, device="cuda", dtype=torch.float64))
+        assert float(res.max()) <= 1e-4, (size, float(res.max()))
+        assert float(orth.abs().amax()) <= 1e-4, size
+        # sampled against LAPACK and the oracle (sorted: the order of a
+        # parallel Jacobi sweep is not numax's row-cyclic order)
+        for p in rng.choice(idx, size=8, replace=False):
+            m32 = a[oa[p]:oa[p] + size * size].reshape(size, size).astype(np.float64)
+            wp = np.sort(w[int(ow[p]):int(ow[p]) + size].cpu().numpy().astype(np.float64))
+            sc = max(np.abs(np.linalg.eigvalsh(m32)).max(), 1e-30)
+            assert np.max(np.abs(wp - np.linalg.eigvalsh(m32))) <= 2e-5 * sc
+            wo, _ = oracle.array_eigh(m32)
+            assert np.max(np.abs(wp - np.sort(wo))) <= 2e-5 * sc
+
+
+def test_c5_lstsq_4096():
+    import torch
+    a, b = R.lstsq_batch(4096, 1024, 32, 6)
+    x = nl.lstsq_batched(a, b)
+    A = torch.as_tensor(a, device="cuda").double()
+    B = torch.as_tensor(b, device="cuda").double()
+    X = x.double()
+    r = B - torch.bmm(A, X[:, :, None])[:, :, 0]
+    ne = torch.bmm(A.transpose(1, 2).contiguous(), r[:, :, None])[:, :, 0].abs().amax(1)
+    bound = 1e-3 * torch.linalg.matrix_norm(A) * torch.linalg.vector_norm(B, dim=1)
+    assert bool((ne <= bound).all()), float((ne / bound).max())
+    rng = np.random.default_rng(8)
+    for p in rng.choice(4096, size=12, replace=False):
+        xo = oracle.array_lstsq(a[p].astype(np.float64), b[p].astype(np.float64))
+        xp = x[p].cpu().numpy().astype(np.float64)
+        assert np.max(np.abs(xp - xo)) <= 1e-4 * max(1.0, np.abs(xo).max()), p
