@@ -202,6 +202,22 @@ int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* o
                      const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems, double* X,
                      int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg);
 
+/* nmx_batched_gesv without allocating: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels and the
+ * same results as nmx_batched_gesv, with the call's scratch -- 4 a_elems
+ * bytes of FP32 working copies, 8 count bytes of offsets and about
+ * 4 (count + 1100) bytes of queue, each rounded up to 256 -- in a 64 MiB
+ * static device array each device holds for the life of the process.  A
+ * call whose scratch exceeds 64 MiB (roughly a_elems + 3 count > 16.7e6)
+ * returns NMX_ERR_UNSUPPORTED and queues nothing.  Every call on one device
+ * uses that device's one array: calls that overlap on a device (different
+ * streams or contexts, not ordered) race on it; each call clears what it
+ * needs cleared. */
+int nmx_batched_gesv_noalloc(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                             const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                             double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg);
+
 /* Autotuner of the launch shape (no reference counterpart): for each class
  * (n > 128, 64 < n <= 128, n <= 64) times a synthetic batch of `samples`
  * problems of that class at 1, 2, 3, 4, 6 and 8 resident workgroups per CU

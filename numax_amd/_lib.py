@@ -71,6 +71,8 @@ SIGNATURES = {
     "nmx_batched_gels_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp]),
     "nmx_batched_gesv": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
                                   _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nmx_batched_gesv_noalloc": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,
+                                          _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_ir_cfg_default": (None, [_c_vp]),
     "nmx_gesv_mpir": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_mp_factor_debug": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_int, _c_int, _c_u64, _c_int, _c_int,

@@ -1173,9 +1173,11 @@ NMX_API void nmx_batch_cfg_default(nmx_batch_cfg* cfg) {
   cfg->teams_per_cu[2] = 4;  // n <= 64
 }
 
-NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
-                             const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
-                             double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg_in) {
+// Arguments of the batched solve, checked once for both entry points.
+static int batched_args(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                        const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                        double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg_in,
+                        nmx_batch_cfg* cfg) {
   if (!ctx) return -1;
   if (count < 0) return -2;
   if (count == 0) return NMX_OK;
@@ -1191,22 +1193,35 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   if (!info) return -12;
   if (!iters) return -13;
   if (!berr) return -14;
-  nmx_batch_cfg cfg;
-  if (cfg_in) cfg = *cfg_in; else nmx_batch_cfg_default(&cfg);
-  if (cfg.max_iter < 0 || !(cfg.tol >= 0.0)) return -15;
+  if (cfg_in) *cfg = *cfg_in; else nmx_batch_cfg_default(cfg);
+  if (cfg->max_iter < 0 || !(cfg->tol >= 0.0)) return -15;
   for (int c = 0; c < 3; c++)
-    if (cfg.teams_per_cu[c] <= 0 || cfg.teams_per_cu[c] > 64) return -15;
+    if (cfg->teams_per_cu[c] <= 0 || cfg->teams_per_cu[c] > 64) return -15;
+  return NMX_OK;
+}
 
-  // workspace: FP32 working copies (as many elements as A), their offsets,
-  // then the queue
-  const size_t f_bytes = ((sizeof(float) * (size_t)a_elems + 255) / 256) * 256;
-  const size_t o_bytes = ((sizeof(int64_t) * (size_t)count + 255) / 256) * 256;
-  const size_t q_ints = 2 * (size_t)kBins + 2 * kClasses + kClasses + (size_t)count;
-  void* ws = nullptr;
-  NMX_TRY(nmx_workspace(ctx, f_bytes + o_bytes + sizeof(int32_t) * q_ints, &ws));
+// Scratch of one call: FP32 working copies (as many elements as A), their
+// offsets, then the size-class queue.
+struct BatchWs {
+  size_t f_bytes, o_bytes, q_ints;
+  size_t total() const { return f_bytes + o_bytes + sizeof(int32_t) * q_ints; }
+};
+static BatchWs batched_ws(int count, int64_t a_elems) {
+  BatchWs w;
+  w.f_bytes = ((sizeof(float) * (size_t)a_elems + 255) / 256) * 256;
+  w.o_bytes = ((sizeof(int64_t) * (size_t)count + 255) / 256) * 256;
+  w.q_ints = 2 * (size_t)kBins + 2 * kClasses + kClasses + (size_t)count;
+  return w;
+}
+
+// The solve itself, in the scratch `ws` (batched_ws bytes) the caller found.
+static int batched_run(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                       const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                       double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg& cfg, void* ws) {
+  const BatchWs wl = batched_ws(count, a_elems);
   float* F = (float*)ws;
-  int64_t* offF = (int64_t*)((char*)ws + f_bytes);
-  int32_t* hist = (int32_t*)((char*)ws + f_bytes + o_bytes);
+  int64_t* offF = (int64_t*)((char*)ws + wl.f_bytes);
+  int32_t* hist = (int32_t*)((char*)ws + wl.f_bytes + wl.o_bytes);
   int32_t* cursor = hist + kBins;
   int32_t* cls = cursor + kBins;
   int32_t* head = cls + 2 * kClasses;
@@ -1236,6 +1251,31 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
     NMX_TRY((launch_class<1, true>(ctx, a, cfg.teams_per_cu[2])));
   }
   return NMX_OK;
+}
+
+NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                             const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                             double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg* cfg_in) {
+  nmx_batch_cfg cfg;
+  NMX_TRY(batched_args(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg_in, &cfg));
+  if (count == 0) return NMX_OK;
+  void* ws = nullptr;
+  NMX_TRY(nmx_workspace(ctx, batched_ws(count, a_elems).total(), &ws));
+  return batched_run(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg, ws);
+}
+
+NMX_API int nmx_batched_gesv_noalloc(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA,
+                                     const int64_t* offB, const uint8_t* kind, const double* A, int64_t a_elems,
+                                     const double* B, int64_t b_elems, double* X, int32_t* info, int32_t* iters,
+                                     double* berr, const nmx_batch_cfg* cfg_in) {
+  nmx_batch_cfg cfg;
+  NMX_TRY(batched_args(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg_in, &cfg));
+  if (count == 0) return NMX_OK;
+  // the context's device copy of the static scratch array: never grown
+  if (!ctx->static_scratch || batched_ws(count, a_elems).total() > ctx->static_scratch_bytes)
+    return NMX_ERR_UNSUPPORTED;
+  return batched_run(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg,
+                     ctx->static_scratch);
 }
 
 // ---- autotuner: resident workgroups per CU, per size class ---------------------

@@ -20,6 +20,14 @@ NMX_API const char* nmx_status_string(int status) {
   }
 }
 
+// Scratch of the allocation-free entry points (nmx_batched_gesv_noalloc):
+// every device the library runs on holds its own copy from module load to
+// process end.  Host code never uses this name as an address -- in host code
+// it is a stand-in, not device memory: nmx_ctx_create takes this device's
+// address of it with hipGetSymbolAddress and keeps it in the context.
+constexpr size_t kStaticScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -61,6 +69,16 @@ NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
   c->mailbox_bytes = kMailboxBytes;
   // an error another runtime user of this process left behind is not ours
   (void)hipGetLastError();
+  // this device's copy of the static scratch array (device `device` is current)
+  void* sp = nullptr;
+  const hipError_t es = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (es != hipSuccess || !sp) {
+    fprintf(stderr, "nmx_ctx_create: hipGetSymbolAddress(scratch) on device %d: %s\n", device, hipGetErrorString(es));
+    nmx_ctx_destroy(c);
+    return NMX_ERR_HIP;
+  }
+  c->static_scratch = sp;
+  c->static_scratch_bytes = kStaticScratchBytes;
   size_t words = kMailboxBytes / 4;
   fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, c->stream>>>((uint32_t*)c->mailbox, words, 0u);
   fill_u32<<<1, 64, 0, c->stream>>>((uint32_t*)c->err, 64, 0u);

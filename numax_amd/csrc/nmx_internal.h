@@ -79,6 +79,11 @@ struct nmx_ctx {
   std::vector<void*> retired;
   // orders a newly set stream after the work queued on the previous one
   hipEvent_t switch_ev = nullptr;
+  // this device's copy of the library's static scratch array (the device
+  // address hipGetSymbolAddress returned at context creation) and its size:
+  // the allocation-free entry points work in it
+  void* static_scratch = nullptr;
+  size_t static_scratch_bytes = 0;
   // cross-workgroup mailbox for the cooperative panel: flags then slots
   void* mailbox = nullptr;
   size_t mailbox_bytes = 0;
