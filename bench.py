@@ -165,7 +165,235 @@ def batched_leg(args, torch) -> dict:
            "teams_per_cu": [int(cfg.teams_per_cu[i]) for i in range(3)], "autotuned": teams is not None,
            "max_backward_error": float(np.nanmax(be)), "max_iterations": int(it.max()),
            "failed": int((inf != 0).sum()), "kernel_profile": prof}
+    if not args.no_cpu:
+        import oracle
+        threads, vis = host_threads()
+        oracle.set_threads(threads)
+        sa, sb, sn, skind, soa, sob = R.device_uneven_batch(args.batch_cpu_sample, 4)
+        sa, sb = sa.cpu().numpy(), sb.cpu().numpy()
+        t0 = time.perf_counter()
+        oracle.array_gesv_batch(sa, sb, sn, skind, soa, sob)
+        dts = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(sn) / dts, "unit": "systems/s", "cores": threads, "cores_visible": vis,
+                               "kind": "port",
+                               "sample": f"oracle register-tier lu_factor + PivotedLU.solve / cholesky + "
+                                         f"cholesky_solve (FP64, one system per thread), {len(sn)} systems of the "
+                                         f"same distribution, {dts:.2f} s"}
     log("batched: " + json.dumps({k: v for k, v in out.items() if k != "kernel_profile"}))
+    return out
+
+
+def host_threads() -> tuple[int, int]:
+    """(threads used, cores visible): the CPU baselines use every core this
+    process may run on, at most 16 (a one-GPU box's share of the host)."""
+    vis = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return min(vis, 16), vis
+
+
+def timed_steps(step, steps, warmup, torch):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def class_roofline(prof, name, bound, flops=None, nbytes=None, peak_key=None) -> dict:
+    """Roofline object of one kernel class from the library's per-launch HIP
+    events: `flops` / `nbytes` are the algorithmic work of the whole step
+    (None: the profiler's own per-launch counts)."""
+    rec = prof[name]
+    sec = rec["ms"] * 1e-3
+    if bound == "mfma" or bound == "valu":
+        work = rec["flops"] if flops is None else flops
+        ach = work / sec / 1e12
+        peak = PEAK[peak_key]
+        unit = "TFLOP/s"
+    else:
+        work = rec["bytes"] if nbytes is None else nbytes
+        ach = work / sec / 1e9
+        peak = PEAK["hbm_GBs"]
+        unit = "GB/s"
+    return {"kernel": name, "bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
+            "traffic": None, "launches_per_step": rec["launches"], "kernel_ms": rec["ms"],
+            "avg_launch_ms": rec["ms"] / max(rec["launches"], 1)}
+
+
+def c2_leg(args, torch) -> dict:
+    """Plain FP64 GPU LU (A ~ N(0,1)) and Cholesky (A = G G^T / n + I),
+    n = 8192, 16 right-hand sides ~ N(0,1), block 256: factor + solve per
+    step, the FP64 MFMA trailing update's roofline, the oracle beside it."""
+    import ctypes
+
+    import numpy as np
+
+    from numax_amd import _lib
+    from numax_amd import linalg as nl
+    from numax_amd import random as R
+    n, k = args.c2_n, 16
+    lib = _lib.load()
+    a_lu = R.device_normal(21, (n, n))
+    g = R.device_normal(22, (n, n))
+    a_ch = g @ g.T / n + torch.eye(n, dtype=torch.float64, device="cuda")
+    del g
+    bm = R.device_normal(23, (n, k))
+    work = torch.empty_like(a_lu)
+    x = torch.empty_like(bm)
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx = nl._ctx(work)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def lu_step():
+        work.copy_(a_lu)
+        _lib.check(lib.nmx_getrf_f64(ctx.handle, n, P(work), n, P(piv), P(info), 256, 32), "getrf")
+        x.copy_(bm)
+        _lib.check(lib.nmx_getrs_f64(ctx.handle, n, k, P(work), n, P(piv), P(x), k), "getrs")
+
+    def chol_step():
+        work.copy_(a_ch)
+        _lib.check(lib.nmx_potrf_f64(ctx.handle, n, P(work), n, P(info), 256), "potrf")
+        x.copy_(bm)
+        _lib.check(lib.nmx_potrs_f64(ctx.handle, n, k, P(work), n, P(x), k), "potrs")
+
+    out = {}
+    threads, vis = host_threads()
+    for name, step, a, fl, cpu_fn in (("lu", lu_step, a_lu, 2 * n ** 3 / 3 + 2 * n * n * k, "lu"),
+                                      ("cholesky", chol_step, a_ch, n ** 3 / 3 + 2 * n * n * k, "chol")):
+        dt = timed_steps(step, args.steps, 1, torch)
+        eta = backward_error(a, x, bm)
+        prof = profile_step(ctx, step)
+        roof = class_roofline(prof, "gemm", "mfma", peak_key="fp64_TFs")
+        roof["share_of_step"] = prof["gemm"]["ms"] / (dt * 1e3)
+        cpu = None
+        if not args.no_cpu:
+            cpu = c2_cpu_baseline(cpu_fn, args.c2_cpu_sample, n, k, threads, vis)
+        out[name] = {"workload": f"FP64 {'LU with partial pivoting' if name == 'lu' else 'Cholesky'} factor + "
+                                 f"solve, n={n}, {k} RHS, block 256",
+                     "ms_per_step": dt * 1e3, "solves_per_s": k / dt, "tflops": fl / dt / 1e12,
+                     "backward_error": eta, "roofline": roof, "cpu_baseline": cpu, "kernel_profile": prof}
+        log(f"c2 {name}: {dt * 1e3:.2f} ms, {fl / dt / 1e12:.2f} TFLOP/s, eta {eta:.2e}, gemm "
+            f"{roof['achieved']:.1f} TFLOP/s")
+    del a_lu, a_ch, work
+    torch.cuda.empty_cache()
+    return out
+
+
+def c2_cpu_baseline(kind: str, n_sample: int, n: int, k: int, threads: int, vis: int) -> dict:
+    """The oracle (numax's host LU, block 32 / base 16, or Cholesky, block
+    64 / tile 128, with its threaded trsm / GEMM) on an n_sample problem,
+    scaled to n by flop count."""
+    import numpy as np
+
+    import oracle
+    from numax_amd import random as R
+    oracle.set_threads(threads)
+    b = R.normal(7, (n_sample, k))
+    if kind == "lu":
+        a = R.normal(8, (n_sample, n_sample))
+        t0 = time.perf_counter()
+        lu, piv, _, _ = oracle.getrf(a)
+        oracle.getrs(lu, piv, b)
+        dt = time.perf_counter() - t0
+        fl_s, fl = 2 * n_sample ** 3 / 3 + 2 * n_sample ** 2 * k, 2 * n ** 3 / 3 + 2 * n * n * k
+    else:
+        a = R.spd_matrix(9, n_sample)
+        t0 = time.perf_counter()
+        l, _ = oracle.potrf(a)
+        oracle.potrs(l, b)
+        dt = time.perf_counter() - t0
+        fl_s, fl = n_sample ** 3 / 3 + 2 * n_sample ** 2 * k, n ** 3 / 3 + 2 * n * n * k
+    rate = fl_s / dt
+    return {"value": rate / fl * k, "unit": "solves/s", "cores": threads, "cores_visible": vis, "kind": "port",
+            "sample": f"oracle {kind} factor + solve, n={n_sample}, {k} RHS, f64: {dt:.2f} s -> "
+                      f"{rate / 1e9:.1f} GFLOP/s, scaled to n={n} by flop count"}
+
+
+def c5_legs(args, torch) -> dict:
+    """Batched Jacobi eigh (65,536 matrices, n in {24, 32, 48, 64}) and
+    batched tall-skinny least squares (4096 problems 1024 x 32), FP32."""
+    import ctypes
+
+    import numpy as np
+
+    import oracle
+    from numax_amd import _lib
+    from numax_amd import linalg as nl
+    from numax_amd import random as R
+    lib = _lib.load()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    threads, vis = host_threads()
+    out = {}
+    # ---- eigh
+    a, n, oa, ow = R.eigh_batch(args.eigh_count, 5)
+    A = torch.as_tensor(a, device="cuda")
+    N = torch.as_tensor(n, device="cuda")
+    OA = torch.as_tensor(oa, device="cuda")
+    OW = torch.as_tensor(ow, device="cuda")
+    W = torch.zeros(int((ow + n).max()), dtype=torch.float32, device="cuda")
+    V = torch.zeros_like(A)
+    ctx = nl._ctx(A)
+
+    def eigh_step():
+        _lib.check(lib.nmx_batched_syevj_f32(ctx.handle, len(n), P(N), P(OA), P(OW), P(A), A.numel(), P(W), W.numel(),
+                                             P(V), 12), "eigh")
+
+    dt = timed_steps(eigh_step, args.steps, 1, torch)
+    prof = profile_step(ctx, eigh_step)
+    nn = n.astype(np.float64)
+    flops = float((12 * nn * (nn - 1) / 2 * (18 * nn + 15)).sum())  # numax's rotation count x flops
+    roof = class_roofline(prof, "batched_eigh", "valu", flops=flops, peak_key="fp32_TFs")
+    cpu = None
+    if not args.no_cpu:
+        oracle.set_threads(threads)
+        sample = np.concatenate([np.flatnonzero(n == s)[:args.eigh_cpu_sample // 4] for s in (24, 32, 48, 64)])
+        sa = np.concatenate([a[oa[p]:oa[p] + n[p] * n[p]] for p in sample]).astype(np.float64)
+        sn = n[sample]
+        soa = np.zeros(len(sample), dtype=np.int64)
+        soa[1:] = np.cumsum(sn.astype(np.int64) ** 2)[:-1]
+        sow = np.zeros(len(sample), dtype=np.int64)
+        sow[1:] = np.cumsum(sn)[:-1]
+        t0 = time.perf_counter()
+        oracle.array_eigh_batch(sa, sn, soa, sow)
+        dts = time.perf_counter() - t0
+        cpu = {"value": len(sample) / dts, "unit": "matrices/s", "cores": threads, "cores_visible": vis,
+               "kind": "port", "sample": f"oracle register-tier eigh (12 cyclic Jacobi sweeps, FP64), "
+                                         f"{len(sample)} of the matrices (equal shares of each n), {dts:.2f} s"}
+    out["eigh"] = {"workload": f"{len(n)} symmetric eigenproblems, n in {{24, 32, 48, 64}} in equal shares, half "
+                               "finite-difference Hamiltonians, half (G + G^T)/2; 12 Jacobi sweeps, FP32, vectors",
+                   "ms_per_step": dt * 1e3, "matrices_per_s": len(n) / dt, "tflops": flops / dt / 1e12,
+                   "roofline": roof, "cpu_baseline": cpu, "kernel_profile": prof}
+    log(f"c5 eigh: {dt * 1e3:.2f} ms, {len(n) / dt:.3g} matrices/s, {roof['achieved']:.2f} TFLOP/s")
+    del A, V
+    # ---- lstsq
+    m, k, cnt = 1024, 32, args.lstsq_count
+    a, b = R.lstsq_batch(cnt, m, k, 6)
+    A = torch.as_tensor(a, device="cuda")
+    B = torch.as_tensor(b, device="cuda")
+    X = torch.zeros((cnt, k), dtype=torch.float32, device="cuda")
+
+    def lstsq_step():
+        _lib.check(lib.nmx_batched_gels_f32(ctx.handle, cnt, m, k, P(A), P(B), P(X)), "gels")
+
+    dt = timed_steps(lstsq_step, args.steps, 1, torch)
+    prof = profile_step(ctx, lstsq_step)
+    nbytes = 4.0 * (m * k + m + k) * cnt
+    roof = class_roofline(prof, "batched_lstsq", "hbm", nbytes=nbytes)
+    cpu = None
+    if not args.no_cpu:
+        s = args.lstsq_cpu_sample
+        t0 = time.perf_counter()
+        oracle.array_lstsq_batch(a[:s], b[:s])
+        dts = time.perf_counter() - t0
+        cpu = {"value": s / dts, "unit": "problems/s", "cores": threads, "cores_visible": vis, "kind": "port",
+               "sample": f"oracle register-tier lstsq (Householder, FP64), {s} of the problems, {dts:.2f} s"}
+    out["lstsq"] = {"workload": f"{cnt} least-squares problems, A {m} x {k} ~ N(0,1), b ~ N(0,1), FP32",
+                    "ms_per_step": dt * 1e3, "problems_per_s": cnt / dt, "roofline": roof, "cpu_baseline": cpu,
+                    "kernel_profile": prof}
+    log(f"c5 lstsq: {dt * 1e3:.3f} ms, {cnt / dt:.3g} problems/s, {roof['achieved']:.0f} GB/s")
     return out
 
 
@@ -188,6 +416,16 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=20000, help="systems in the uneven batch (0: skip it)")
+    ap.add_argument("--pivot", default="tournament", choices=["tournament", "partial"],
+                    help="how the mixed-precision factor pivots")
+    ap.add_argument("--c2-n", type=int, default=8192, help="order of the plain FP64 LU / Cholesky leg (0: skip)")
+    ap.add_argument("--c2-cpu-sample", type=int, default=4096)
+    ap.add_argument("--eigh-count", type=int, default=65536)
+    ap.add_argument("--eigh-cpu-sample", type=int, default=2048)
+    ap.add_argument("--lstsq-count", type=int, default=4096)
+    ap.add_argument("--lstsq-cpu-sample", type=int, default=512)
+    ap.add_argument("--no-c5", action="store_true", help="skip the batched eigh / lstsq legs")
+    ap.add_argument("--batch-cpu-sample", type=int, default=2000)
     ap.add_argument("--batched-only", action="store_true", help="time only the uneven batch (prints its object)")
     ap.add_argument("--matrix", default="haar", choices=["haar", "normal"],
                     help="haar: U diag(sigma) V^T (the workload); normal: N(0,1) entries (no QR in the setup, for "
@@ -255,6 +493,7 @@ def main() -> int:
     cfg = _lib.IrCfg()
     lib.nmx_ir_cfg_default(ctypes.byref(cfg))
     cfg.precision = nl.PRECISIONS[args.precision]
+    cfg.pivot = nl.PIVOTS[args.pivot]
     cfg.block = args.block
     if not args.no_autotune:
         lib_st = lib.nmx_ir_autotune(ctx.handle, n, ctypes.byref(cfg))
@@ -347,13 +586,17 @@ def main() -> int:
     roof_plain["share_of_step"] = prof_plain[roof_plain["kernel"]]["ms"] / (dt_plain * 1e3)
 
     batched = batched_leg(args, torch) if args.batch_count > 0 else None
+    del a, work
+    torch.cuda.empty_cache()
+    c2 = c2_leg(args, torch) if args.c2_n > 0 else None
+    c5 = None if args.no_c5 else c5_legs(args, torch)
 
     cpu = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = min(threads, 16)
-        log(f"cpu baseline: oracle LU n={args.cpu_sample} on {threads} threads")
+        threads, vis = host_threads()
+        log(f"cpu baseline: oracle LU n={args.cpu_sample} on {threads} threads ({vis} visible)")
         cpu = cpu_baseline(args.cpu_sample, threads)
+        cpu["cores_visible"] = vis
 
     value = 1.0 / (head["ms_per_step"] * 1e-3)
     out = {
@@ -370,9 +613,10 @@ def main() -> int:
         "dtype": f"mixed: {args.precision} MFMA update / f32 panel / f64 residual",
         "data": "synthetic",
         "config": {"workload": f"mixed-precision LU-IR solve, n={n}, 1 RHS, A = U diag(sigma) V^T (Haar U, V), "
-                               f"kappa={args.kappa:g}, partial pivoting, {args.precision} trailing update, "
+                               f"kappa={args.kappa:g}, {args.pivot} pivoting, {args.precision} trailing update, "
                                f"block {mp_block}, refined to FP64 backward error <= {args.tol:g}",
-                   "n": n, "nrhs": 1, "kappa": args.kappa, "precision": args.precision, "block": mp_block,
+                   "n": n, "nrhs": 1, "kappa": args.kappa, "precision": args.precision, "pivot": args.pivot,
+                   "block": mp_block,
                    "autotuned": not args.no_autotune},
         "tflops": head["effective_tflops"],
         "backward_error": head["backward_error"],
@@ -386,6 +630,8 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_uneven": batched,
+        "c2_plain_fp64_n8192": c2,
+        "c5_batched_eigh_lstsq": c5,
         "kernel_profile": prof_mp,
     }
     print(json.dumps(out), flush=True)

@@ -82,6 +82,7 @@ int nmx_getrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* LU, int lda, const
 
 /* laswp / laswp_matrix (numax/linalg/panel.mojo:1824-1852, :2279-2313). */
 int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, const int32_t* ipiv);
+int nmx_laswp_f32(nmx_ctx* ctx, int n, int nrhs, float* B, int ldb, const int32_t* ipiv);
 
 /* LU._diagonal (lu.mojo:355-379): gather diag(A) into an n-vector for
  * det / slogdet. */
@@ -118,11 +119,12 @@ int nmx_gemm_f32(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, floa
 /* ---- mixed-precision LU with iterative refinement --------------------------
  * No reference counterpart (numax has no mixed precision and no refinement);
  * it serves `solve(a, b)` (numax/linalg/basic.mojo:55-146) when a caller asks
- * for FP64 accuracy at low-precision factorization cost.  A' = W^T A V with
- * random butterflies W, V; no-pivot blocked LU of A' in FP32 with the
- * trailing updates on FP16 / BF16 MFMA (FP32 accumulate, power-of-two operand
- * scaling); then r = b - A x in FP64, x += V U^-1 L^-1 W^T r until the
- * normwise backward error ||r||_inf / (||A||_inf ||x||_inf) <= tol.
+ * for FP64 accuracy at low-precision factorization cost.  Blocked LU of A
+ * in FP32 (P A = L U; pivoting per cfg.pivot) with the trailing updates on
+ * FP16 / BF16 MFMA (FP32 accumulate, power-of-two operand scaling); then
+ * r = b - A x in FP64, x += U^-1 L^-1 P r (or GMRES on that correction
+ * equation) until the normwise backward error
+ * ||r||_inf / (||A||_inf ||x||_inf) <= tol.
  * A is read, never written; x receives the solution. */
 typedef struct nmx_ir_cfg {
   int precision;     /* trailing update: 0 fp16, 1 bf16, 2 fp16 split (hi + 2^-11 lo, 3 products), 3 fp32 */
@@ -132,7 +134,9 @@ typedef struct nmx_ir_cfg {
   int gmres_restart; /* GMRES restart length */
   double tol;        /* target backward error */
   uint64_t seed;     /* butterfly seed (pivot = 0) */
-  int pivot;         /* 1: partial pivoting (default); 0: random butterflies and no pivoting */
+  int pivot;         /* 2: tournament pivoting (default: all pivots of a 32-column leaf from one
+                        reduction tree, Grigori-Demmel-Xiang CALU); 1: partial pivoting, column by
+                        column (numax's pivots; n <= 32768); 0: random butterflies, no pivoting */
 } nmx_ir_cfg;
 
 typedef struct nmx_ir_report {
@@ -246,7 +250,10 @@ int nmx_batched_syevj_f32(nmx_ctx* ctx, int count, const int32_t* n, const int64
  * b_p (m) and x_p (n), FP32, packed back to back. */
 int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const float* A, const float* B, float* X);
 
-/* ---- Householder QR (FP64, m >= n, row-major) --------------------------------
+/* ---- Householder QR (FP64 and FP32, m >= n, row-major) -----------------------
+ * numax's qr_factor[gpu=True] / lstsq[gpu=True] run at float32 only
+ * (numax's GPU factorizations are float32); the _f32 entry points are those,
+ * the _f64 ones serve the host-precision callers.
  * qr_factor (numax/linalg/qr.mojo:615-743): blocked right-looking geqrf in
  * place -- R on and above the diagonal, the reflectors' V below it (unit
  * diagonal implicit), tau[n] their scales -- with geqr2_panel's dlarfg
@@ -254,20 +261,27 @@ int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const float* A, 
  * (1..64; numax's default is 16) and the panel's block reflector applied
  * to the trailing columns as C -= V (T^T (V^T C)) (qr.mojo:123-327). */
 int nmx_geqrf_f64(nmx_ctx* ctx, int m, int n, double* A, int lda, double* tau, int block);
+int nmx_geqrf_f32(nmx_ctx* ctx, int m, int n, float* A, int lda, float* tau, int block);
 /* QR.apply_q_transpose (qr.mojo:470-534) when trans != 0: B := Q^T B for an
  * m x nrhs B; trans == 0 applies Q. */
 int nmx_ormqr_f64(nmx_ctx* ctx, int trans, int m, int n, int nrhs, const double* QR, int lda, const double* tau,
                   double* B, int ldb, int block);
+int nmx_ormqr_f32(nmx_ctx* ctx, int trans, int m, int n, int nrhs, const float* QR, int lda, const float* tau,
+                  float* B, int ldb, int block);
 /* QR.q (qr.mojo:405-468): the thin m x n Q, the panels applied to the
  * identity last first. */
 int nmx_orgqr_f64(nmx_ctx* ctx, int m, int n, const double* QR, int lda, const double* tau, double* Q, int ldq,
                   int block);
+int nmx_orgqr_f32(nmx_ctx* ctx, int m, int n, const float* QR, int lda, const float* tau, float* Q, int ldq,
+                  int block);
 /* QR.r (qr.mojo:371-403): the n x n upper triangle, zeros below. */
 int nmx_qr_r_f64(nmx_ctx* ctx, int n, const double* QR, int lda, double* R, int ldr);
+int nmx_qr_r_f32(nmx_ctx* ctx, int n, const float* QR, int lda, float* R, int ldr);
 /* lstsq(method "qr") and QR.solve (qr.mojo:746-821, :536-612): factors A in
  * place (tau[n]), then B (m x nrhs) := Q^T B and R X = B[0:n] solved in
  * place -- X is B's first n rows. */
 int nmx_gels_f64(nmx_ctx* ctx, int m, int n, int nrhs, double* A, int lda, double* tau, double* B, int ldb, int block);
+int nmx_gels_f32(nmx_ctx* ctx, int m, int n, int nrhs, float* A, int lda, float* tau, float* B, int ldb, int block);
 
 /* ---- graph capture ----------------------------------------------------------
  * Records the library calls made on the context's stream between

@@ -39,6 +39,7 @@ SIGNATURES = {
     "nmx_getrs_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int]),
     "nmx_getrs_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int]),
     "nmx_laswp_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp]),
+    "nmx_laswp_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp]),
     "nmx_diag_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp]),
     "nmx_diag_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp]),
     "nmx_trsm_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
@@ -56,10 +57,15 @@ SIGNATURES = {
     "nmx_batch_autotune": (_c_int, [_c_vp, _c_int, _c_vp]),
     "nmx_ir_autotune": (_c_int, [_c_vp, _c_int, _c_vp]),
     "nmx_geqrf_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_geqrf_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_ormqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
+    "nmx_ormqr_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_orgqr_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
+    "nmx_orgqr_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_qr_r_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
+    "nmx_qr_r_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_int]),
     "nmx_gels_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
+    "nmx_gels_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_int]),
     "nmx_gemm_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_vp, _c_int, _c_vp,
                               _c_int, _c_dbl, _c_vp, _c_int]),
     "nmx_gemm_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_flt, _c_vp, _c_int, _c_vp,

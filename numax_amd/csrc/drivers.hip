@@ -189,7 +189,8 @@ NMX_API int nmx_getrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* LU, int ld
   return getrs_api<float>(ctx, n, nrhs, LU, lda, ipiv, B, ldb);
 }
 
-NMX_API int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, const int32_t* ipiv) {
+template <typename T>
+static int laswp_api(nmx_ctx* ctx, int n, int nrhs, T* B, int ldb, const int32_t* ipiv) {
   if (!ctx) return -1;
   if (n < 0) return -2;
   if (nrhs < 0) return -3;
@@ -197,7 +198,13 @@ NMX_API int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, con
   if (n == 0 || nrhs == 0) return NMX_OK;
   if (!B) return -4;
   if (!ipiv) return -6;
-  return laswp_full<double>(ctx, n, nrhs, B, ldb, ipiv);
+  return laswp_full<T>(ctx, n, nrhs, B, ldb, ipiv);
+}
+NMX_API int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, const int32_t* ipiv) {
+  return laswp_api<double>(ctx, n, nrhs, B, ldb, ipiv);
+}
+NMX_API int nmx_laswp_f32(nmx_ctx* ctx, int n, int nrhs, float* B, int ldb, const int32_t* ipiv) {
+  return laswp_api<float>(ctx, n, nrhs, B, ldb, ipiv);
 }
 
 template <typename T>

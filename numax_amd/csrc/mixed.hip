@@ -1678,7 +1678,7 @@ NMX_API void nmx_ir_cfg_default(nmx_ir_cfg* cfg) {
   cfg->gmres_restart = 30;
   cfg->tol = 1e-15;
   cfg->seed = 0x5eedULL;
-  cfg->pivot = 1;
+  cfg->pivot = 2;
 }
 
 // Test hook: the transformed matrix (stage 0) or its no-pivot factors

@@ -179,14 +179,14 @@ def lu_factor(a, block: int = 16, base: int = 16) -> LU:
 def laswp(b, pivots) -> torch.Tensor:
     """laswp / laswp_matrix (numax/linalg/panel.mojo:1824-1852, :2279-2313):
     apply the interchanges ``pivots`` (LAPACK order, 0-based) to the rows of a
-    float64 vector or n x k matrix ``b``; returns the permuted copy."""
-    x = _device_tensor(b, torch.float64).clone()
+    float64 or float32 vector or n x k matrix ``b``; returns the permuted copy."""
+    x = _device_tensor(b).clone()
     n = x.shape[0]
     nrhs = 1 if x.dim() == 1 else x.shape[1]
     piv = _device_tensor(pivots, torch.int32)
     if piv.numel() != n:
         raise NumaxError(f"laswp: {piv.numel()} pivots for {n} rows")
-    _lib.check(_lib.load().nmx_laswp_f64(_ctx(x).handle, n, nrhs, _ptr(x), max(nrhs, 1), _ptr(piv)), "laswp")
+    _lib.check(_fn("laswp", x.dtype)(_ctx(x).handle, n, nrhs, _ptr(x), max(nrhs, 1), _ptr(piv)), "laswp")
     return x
 
 
@@ -265,27 +265,26 @@ class QR:
     def r(self) -> torch.Tensor:
         """The ``n x n`` upper-triangular R, zeros below (qr.mojo:371-403)."""
         n = self.n
-        out = torch.empty((n, n), dtype=torch.float64, device=self.factored.device)
-        _lib.check(_lib.load().nmx_qr_r_f64(_ctx(out).handle, n, _ptr(self.factored), max(n, 1), _ptr(out),
-                                            max(n, 1)), "QR.r")
+        out = torch.empty((n, n), dtype=self.factored.dtype, device=self.factored.device)
+        _lib.check(_fn("qr_r", out.dtype)(_ctx(out).handle, n, _ptr(self.factored), max(n, 1), _ptr(out), max(n, 1)),
+                   "QR.r")
         return out
 
     def q(self) -> torch.Tensor:
         """The thin ``m x n`` Q with orthonormal columns (qr.mojo:405-468)."""
         m, n = self.m, self.n
-        out = torch.empty((m, n), dtype=torch.float64, device=self.factored.device)
-        _lib.check(_lib.load().nmx_orgqr_f64(_ctx(out).handle, m, n, _ptr(self.factored), max(n, 1),
-                                             _ptr(self.taus), _ptr(out), max(n, 1), self.block), "QR.q")
+        out = torch.empty((m, n), dtype=self.factored.dtype, device=self.factored.device)
+        _lib.check(_fn("orgqr", out.dtype)(_ctx(out).handle, m, n, _ptr(self.factored), max(n, 1), _ptr(self.taus),
+                                           _ptr(out), max(n, 1), self.block), "QR.q")
         return out
 
     def _apply(self, b, trans: int, who: str) -> torch.Tensor:
-        x = _device_tensor(b, torch.float64).clone()
+        x = _device_tensor(b, self.factored.dtype).clone()
         if x.shape[0] != self.m:
             raise NumaxError(f"{who}: b has {x.shape[0]} rows for an m = {self.m} factorization")
         nrhs = 1 if x.dim() == 1 else int(x.shape[1])
-        _lib.check(_lib.load().nmx_ormqr_f64(_ctx(x).handle, trans, self.m, self.n, nrhs, _ptr(self.factored),
-                                             max(self.n, 1), _ptr(self.taus), _ptr(x), max(nrhs, 1), self.block),
-                   who)
+        _lib.check(_fn("ormqr", x.dtype)(_ctx(x).handle, trans, self.m, self.n, nrhs, _ptr(self.factored),
+                                         max(self.n, 1), _ptr(self.taus), _ptr(x), max(nrhs, 1), self.block), who)
         return x
 
     def apply_q_transpose(self, b) -> torch.Tensor:
@@ -299,15 +298,15 @@ class QR:
         n = self.n
         nrhs = 1 if y.dim() == 1 else int(y.shape[1])
         x = y[:n].clone()
-        _lib.check(_lib.load().nmx_trsm_f64(_ctx(x).handle, 1, 0, 0, n, nrhs, _ptr(self.factored), max(n, 1),
-                                            _ptr(x), max(nrhs, 1)), "QR.solve")
+        _lib.check(_fn("trsm", x.dtype)(_ctx(x).handle, 1, 0, 0, n, nrhs, _ptr(self.factored), max(n, 1), _ptr(x),
+                                        max(nrhs, 1)), "QR.solve")
         return x
 
 
 def _qr_input(a, who: str) -> torch.Tensor:
     t = _device_tensor(a)
-    if t.dtype != torch.float64:
-        raise NumaxError(f"{who}: dtype {t.dtype} is not supported by the device QR (float64)")
+    if t.dtype not in (torch.float64, torch.float32):
+        raise NumaxError(f"{who}: dtype {t.dtype} is not supported by the device QR (float32, float64)")
     if t.dim() != 2 or t.shape[0] < t.shape[1]:
         raise NumaxError(f"{who}: needs an m x n matrix with m >= n, got shape {tuple(t.shape)}")
     return t
@@ -315,12 +314,12 @@ def _qr_input(a, who: str) -> torch.Tensor:
 
 def qr_factor(a, block: int = 16) -> QR:
     """``qr_factor`` (numax/linalg/qr.mojo:615-743): blocked Householder QR of
-    an ``m x n`` float64 matrix with ``m >= n``; ``block`` is the panel width
-    (1..64, numax's default 16)."""
+    an ``m x n`` float32 (numax's device precision) or float64 matrix with
+    ``m >= n``; ``block`` is the panel width (1..64, numax's default 16)."""
     work = _qr_input(a, "qr_factor").clone()
     m, n = work.shape
-    taus = torch.zeros(n, dtype=torch.float64, device=work.device)
-    _lib.check(_lib.load().nmx_geqrf_f64(_ctx(work).handle, m, n, _ptr(work), max(n, 1), _ptr(taus), block),
+    taus = torch.zeros(n, dtype=work.dtype, device=work.device)
+    _lib.check(_fn("geqrf", work.dtype)(_ctx(work).handle, m, n, _ptr(work), max(n, 1), _ptr(taus), block),
                "qr_factor")
     return QR(work, taus, block)
 
@@ -333,13 +332,13 @@ def lstsq(a, b, block: int = 16, method: str = "qr") -> torch.Tensor:
         raise NumaxError(f"lstsq: method {method!r} is not on this device path (\"qr\")")
     work = _qr_input(a, "lstsq").clone()
     m, n = work.shape
-    x = _device_tensor(b, torch.float64).clone()
+    x = _device_tensor(b, work.dtype).clone()
     if x.shape[0] != m:
         raise NumaxError(f"lstsq: b has {x.shape[0]} rows for an {m} x {n} matrix")
     nrhs = 1 if x.dim() == 1 else int(x.shape[1])
-    taus = torch.zeros(n, dtype=torch.float64, device=work.device)
-    _lib.check(_lib.load().nmx_gels_f64(_ctx(work).handle, m, n, nrhs, _ptr(work), max(n, 1), _ptr(taus), _ptr(x),
-                                        max(nrhs, 1), block), "lstsq")
+    taus = torch.zeros(n, dtype=work.dtype, device=work.device)
+    _lib.check(_fn("gels", work.dtype)(_ctx(work).handle, m, n, nrhs, _ptr(work), max(n, 1), _ptr(taus), _ptr(x),
+                                       max(nrhs, 1), block), "lstsq")
     return x[:n].clone()
 
 
@@ -368,14 +367,17 @@ def _report(rep) -> dict:
 
 
 def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block=None,
-               seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "partial", solver: str = "auto",
+               seed: int = 0x5EED, raise_on_failure: bool = True, pivot: str = "tournament", solver: str = "auto",
                gmres_restart: int = 30):
     """``x`` with ``A x = b`` to FP64 backward error from a low-precision
     factorization (no reference counterpart; numax's ``solve`` is the FP64
     path).  ``precision`` picks the trailing-update arithmetic: ``"fp16"``,
     ``"bf16"``, ``"fp16s"`` (FP16 hi + lo split, ~FP32 accuracy) or ``"fp32"``.
     ``block``: the panel width (a multiple of 64; None: the autotuned width
-    cached for this order and precision, else 256).  ``solver``:
+    cached for this order and precision, else 256).  ``pivot``:
+    ``"tournament"`` (every 32-column leaf's pivots from one reduction tree),
+    ``"partial"`` (column by column, numax's pivot sequence) or
+    ``"butterfly"`` (random butterflies, no pivoting).  ``solver``:
     ``"classical"`` refinement, ``"gmres"`` (GMRES-IR: FP64 GMRES
     on the correction equation, preconditioned by the low-precision factor)
     or ``"auto"`` (classical while each step at least halves the backward

@@ -24,6 +24,7 @@ _LIB_PATH = os.path.join(_HERE, "lib", "libnmx_oracle.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _i = ctypes.c_int
+_vp = ctypes.c_void_p
 
 
 def build() -> str:
@@ -37,6 +38,9 @@ def _load() -> ctypes.CDLL:
         build()
     lib = ctypes.CDLL(_LIB_PATH)
     sig = {
+        "orc_array_gesv_batch": (None, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_array_eigh_batch": (None, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+        "orc_array_lstsq_batch": (None, [_i, _i, _i, _vp, _vp, _vp]),
         "orc_set_threads": (_i, [_i]),
         "orc_gemm_sub": (None, [_i, _i, _i, _dp, _i, _dp, _i, _i, _dp, _i]),
         "orc_getrf": (_i, [_i, _dp, _i, _ip, _ip, _i, _i]),
@@ -76,6 +80,11 @@ def _p(x: np.ndarray):
     if x.dtype == np.int32:
         return x.ctypes.data_as(_ip)
     return x.ctypes.data_as(_dp)
+
+
+def _a(x: np.ndarray):
+    """The address of a contiguous array (the batch entry points)."""
+    return ctypes.c_void_p(x.ctypes.data)
 
 
 def set_threads(t: int) -> int:
@@ -240,6 +249,44 @@ def array_lstsq(a, b):
     m, n = a.shape
     x = np.zeros(n, dtype=np.float64)
     _lib.orc_array_lstsq(m, n, _p(a), _p(bb), _p(x))
+    return x
+
+
+def array_gesv_batch(a, b, n, kind, offa, offb):
+    """``count`` uneven systems (kind 0: ``lu_factor`` + ``PivotedLU.solve``,
+    1: ``cholesky`` + ``cholesky_solve``), one problem per thread."""
+    aa = _f64(a)
+    bb = _f64(b)
+    x = np.zeros_like(bb)
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    oa = np.ascontiguousarray(offa, dtype=np.int64)
+    ob = np.ascontiguousarray(offb, dtype=np.int64)
+    kd = np.ascontiguousarray(kind, dtype=np.uint8)
+    _lib.orc_array_gesv_batch(len(nn), _a(nn), _a(oa), _a(ob), _a(kd), _a(aa), _a(bb), _a(x))
+    return x
+
+
+def array_eigh_batch(a, n, offa, offw, sweeps: int = 12):
+    """``count`` register-tier ``eigh`` problems packed back to back, one per
+    thread: ``(w, v)`` laid out like ``a`` (values at offw, vectors at offa)."""
+    aa = _f64(a)
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    ow = np.ascontiguousarray(offw, dtype=np.int64)
+    w = np.zeros(int((ow + nn).max()) if len(nn) else 1, dtype=np.float64)
+    v = np.zeros_like(aa)
+    oa = np.ascontiguousarray(offa, dtype=np.int64)
+    _lib.orc_array_eigh_batch(len(nn), _a(nn), _a(oa), _a(ow), _a(aa), _a(w), _a(v), int(sweeps))
+    return w, v
+
+
+def array_lstsq_batch(a, b):
+    """``count`` register-tier ``lstsq`` problems (a: count x m x n, b: count
+    x m), one per thread: x (count x n)."""
+    aa = _f64(a)
+    bb = _f64(b)
+    count, m, n = aa.shape
+    x = np.zeros((count, n), dtype=np.float64)
+    _lib.orc_array_lstsq_batch(count, m, n, _a(aa), _a(bb), _a(x))
     return x
 
 

@@ -27,6 +27,7 @@
  * (P A = L U, L L^T = A, Q^T Q = I, trace / Frobenius identities for eigh).
  */
 #include <math.h>
+#include <immintrin.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -100,27 +101,61 @@ typedef struct {
   double* c; int ldc;
 } gemm_ctx;
 
+/* One element: c - sum_k a[i,k] b[k,j], products added in ascending k. */
+static inline double gemm_elem(const gemm_ctx* g, int64_t i, int j) {
+  double acc = 0.0;
+  for (int p = 0; p < g->k; p++)
+    acc += AT(g->a, g->lda, i, p) * (g->trans_b ? AT(g->b, g->ldb, j, p) : AT(g->b, g->ldb, p, j));
+  return AT(g->c, g->ldc, i, j) - acc;
+}
+
+/* Rows lo..hi in blocks of 4 rows x 8 columns held in AVX2 registers; every
+ * element keeps gemm_elem's operation order (a separate multiply and add per
+ * k, ascending -- the file is built with -ffp-contract=off), so the results
+ * are the same bits as the scalar loop.  B^T is read through a packed copy
+ * of the k x 8 panel. */
 static void gemm_sub_rows(void* p, int64_t lo, int64_t hi) {
   gemm_ctx* g = (gemm_ctx*)p;
-  enum { JB = 256 };
-  double acc[JB];
-  for (int64_t i = lo; i < hi; i++) {
-    for (int j0 = 0; j0 < g->n; j0 += JB) {
-      int jn = g->n - j0 < JB ? g->n - j0 : JB;
-      for (int j = 0; j < jn; j++) acc[j] = 0.0;
-      for (int p = 0; p < g->k; p++) {
-        double av = AT(g->a, g->lda, i, p);
-        if (g->trans_b) {
-          for (int j = 0; j < jn; j++) acc[j] += av * AT(g->b, g->ldb, j0 + j, p);
-        } else {
-          const double* brow = &AT(g->b, g->ldb, p, j0);
-          for (int j = 0; j < jn; j++) acc[j] += av * brow[j];
+  const int k = g->k;
+  double* bp = g->trans_b ? (double*)malloc(sizeof(double) * 8 * (size_t)(k > 0 ? k : 1)) : NULL;
+  const int n8 = g->n & ~7;
+  for (int j0 = 0; j0 < n8; j0 += 8) {
+    const double* bcol;
+    int ldb8;
+    if (g->trans_b) {
+      for (int q = 0; q < k; q++)
+        for (int j = 0; j < 8; j++) bp[(size_t)q * 8 + j] = AT(g->b, g->ldb, j0 + j, q);
+      bcol = bp;
+      ldb8 = 8;
+    } else {
+      bcol = &AT(g->b, g->ldb, 0, j0);
+      ldb8 = g->ldb;
+    }
+    int64_t i = lo;
+    for (; i + 4 <= hi; i += 4) {
+      __m256d acc[4][2];
+      for (int r = 0; r < 4; r++) acc[r][0] = acc[r][1] = _mm256_setzero_pd();
+      for (int q = 0; q < k; q++) {
+        const __m256d b0 = _mm256_loadu_pd(bcol + (size_t)q * ldb8);
+        const __m256d b1 = _mm256_loadu_pd(bcol + (size_t)q * ldb8 + 4);
+        for (int r = 0; r < 4; r++) {
+          const __m256d av = _mm256_set1_pd(AT(g->a, g->lda, i + r, q));
+          acc[r][0] = _mm256_add_pd(acc[r][0], _mm256_mul_pd(av, b0));
+          acc[r][1] = _mm256_add_pd(acc[r][1], _mm256_mul_pd(av, b1));
         }
       }
-      double* crow = &AT(g->c, g->ldc, i, j0);
-      for (int j = 0; j < jn; j++) crow[j] = crow[j] - acc[j];
+      for (int r = 0; r < 4; r++) {
+        double* crow = &AT(g->c, g->ldc, i + r, j0);
+        _mm256_storeu_pd(crow, _mm256_sub_pd(_mm256_loadu_pd(crow), acc[r][0]));
+        _mm256_storeu_pd(crow + 4, _mm256_sub_pd(_mm256_loadu_pd(crow + 4), acc[r][1]));
+      }
     }
+    for (; i < hi; i++)
+      for (int j = j0; j < j0 + 8; j++) AT(g->c, g->ldc, i, j) = gemm_elem(g, i, j);
   }
+  for (int64_t i = lo; i < hi; i++)
+    for (int j = n8; j < g->n; j++) AT(g->c, g->ldc, i, j) = gemm_elem(g, i, j);
+  free(bp);
 }
 
 void orc_gemm_sub(int m, int n, int k, const double* a, int lda, const double* b, int ldb, int trans_b,
@@ -743,6 +778,75 @@ void orc_array_lstsq(int m, int n, const double* a, const double* b, double* x) 
     x[i] = total / guard_nonzero(r[i * n + i], PIVOT_FLOOR);
   }
   free(r); free(y); free(v);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Batches: map_blocks' CPU form (core/functional.mojo:691-834) runs one    */
+/* problem per lane over all cores; here one problem per loop index over    */
+/* the threads of parallel_for.  Each problem is exactly the single-problem */
+/* function above.                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  const int32_t* n; const int64_t* offa; const int64_t* offb; const uint8_t* kind;
+  const double* a; const double* b; double* x;
+} gesv_batch_ctx;
+
+static void gesv_batch_range(void* p, int64_t lo, int64_t hi) {
+  gesv_batch_ctx* g = (gesv_batch_ctx*)p;
+  for (int64_t q = lo; q < hi; q++) {
+    const int n = g->n[q];
+    double* f = (double*)malloc(sizeof(double) * (size_t)n * n);
+    if (g->kind[q] == 0) {
+      int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * n);
+      memcpy(f, g->a + g->offa[q], sizeof(double) * (size_t)n * n);
+      orc_array_lu_factor(n, f, perm);
+      orc_array_lu_solve(n, f, perm, g->b + g->offb[q], g->x + g->offb[q]);
+      free(perm);
+    } else {
+      orc_array_cholesky(n, g->a + g->offa[q], f);
+      orc_array_cholesky_solve(n, f, g->b + g->offb[q], g->x + g->offb[q]);
+    }
+    free(f);
+  }
+}
+
+/* count uneven systems, kind 0 = lu_factor + PivotedLU.solve, 1 = cholesky +
+ * cholesky_solve, packed back to back at element offsets offa / offb. */
+void orc_array_gesv_batch(int count, const int32_t* n, const int64_t* offa, const int64_t* offb,
+                          const uint8_t* kind, const double* a, const double* b, double* x) {
+  gesv_batch_ctx g = {n, offa, offb, kind, a, b, x};
+  parallel_for(count, (int64_t)count << 20, gesv_batch_range, &g);
+}
+
+typedef struct {
+  const int32_t* n; const int64_t* offa; const int64_t* offw;
+  const double* a; double* w; double* v; int sweeps;
+} eigh_batch_ctx;
+
+static void eigh_batch_range(void* p, int64_t lo, int64_t hi) {
+  eigh_batch_ctx* g = (eigh_batch_ctx*)p;
+  for (int64_t q = lo; q < hi; q++)
+    orc_array_eigh(g->n[q], g->a + g->offa[q], g->w + g->offw[q], g->v + g->offa[q], g->sweeps);
+}
+
+void orc_array_eigh_batch(int count, const int32_t* n, const int64_t* offa, const int64_t* offw, const double* a,
+                          double* w, double* v, int sweeps) {
+  eigh_batch_ctx g = {n, offa, offw, a, w, v, sweeps};
+  parallel_for(count, (int64_t)count << 20, eigh_batch_range, &g);
+}
+
+typedef struct { int m, n; const double* a; const double* b; double* x; } lstsq_batch_ctx;
+
+static void lstsq_batch_range(void* p, int64_t lo, int64_t hi) {
+  lstsq_batch_ctx* g = (lstsq_batch_ctx*)p;
+  for (int64_t q = lo; q < hi; q++)
+    orc_array_lstsq(g->m, g->n, g->a + q * (int64_t)g->m * g->n, g->b + q * (int64_t)g->m, g->x + q * (int64_t)g->n);
+}
+
+/* count problems m x n packed back to back (A), b of m, x of n each. */
+void orc_array_lstsq_batch(int count, int m, int n, const double* a, const double* b, double* x) {
+  lstsq_batch_ctx g = {m, n, a, b, x};
+  parallel_for(count, (int64_t)count << 20, lstsq_batch_range, &g);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -69,17 +69,20 @@ def test_c3_mpir_64_rhs(haar_pair):
     assert _eta(A, X, B) <= 1e-15
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_c3_single_product_kappa_1e2(haar_pair, precision):
+@pytest.mark.parametrize("precision,pivot", [("fp16", "tournament"), ("bf16", "partial")])
+def test_c3_single_product_kappa_1e2(haar_pair, precision, pivot):
+    # one BF16 product per update (8-bit mantissa) converges at n = 16384 only
+    # with column-by-column pivots; with tournament pivots it stalls (covered
+    # by the honest-report test below)
     A = R.device_svd_matrix(N3, 1e2, factors=haar_pair)
     b = R.device_normal(4, (N3,))
-    x, rep = nl.solve_mpir(A, b, precision=precision, max_iter=30)
+    x, rep = nl.solve_mpir(A, b, precision=precision, max_iter=30, pivot=pivot)
     assert rep["converged"], rep
     assert _eta(A, x, b) <= 1e-15
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
-@pytest.mark.parametrize("kappa", [1e4, 1e6])
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
 def test_c3_single_product_reports_honestly(haar_pair, precision, kappa):
     # a single FP16 / BF16 product per update: whatever the refinement
     # reaches, the report's backward error is the one the FP64 check measures

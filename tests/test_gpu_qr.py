@@ -108,4 +108,56 @@ def test_shape_errors():
     with pytest.raises(nl.NumaxError):
         nl.lstsq(np.ones((6, 3)), np.ones(5))
     with pytest.raises(nl.NumaxError):
-        nl.qr_factor(torch.ones((6, 3), dtype=torch.float32))
+        nl.qr_factor(torch.ones((6, 3), dtype=torch.float16))
+
+
+# ---- FP32: numax's device QR runs at float32 (qr.mojo:615-743, lstsq :746-821).
+# Checked against the FP64 oracle at FP32 tolerances, tolerances stated here:
+# R and the reflectors within 2e-5 * max|A| (scaled by sqrt(m) for the
+# panel's reductions), Q^T Q = I and Q R = A within 1e-5 relative, the
+# least-squares x within 1e-4 relative of the oracle's on well-conditioned
+# N(0,1) problems, and its residual orthogonal to range(A).
+@pytest.mark.parametrize("m,n,block", [(64, 64, 16), (300, 120, 16), (1000, 257, 32), (513, 64, 64), (37, 5, 4)])
+def test_qr_factor_f32_matches_oracle(m, n, block):
+    a = R.normal(m + n, (m, n))
+    f = nl.qr_factor(a.astype(np.float32), block=block)
+    assert f.factored.dtype == torch.float32
+    qr_o, tau_o = oracle.geqrf(a.astype(np.float32).astype(np.float64), block=block)
+    tol = 2e-5 * np.sqrt(m) * np.abs(a).max()
+    np.testing.assert_allclose(f.factored.cpu().numpy(), qr_o, atol=tol)
+    np.testing.assert_allclose(f.taus.cpu().numpy(), tau_o, atol=1e-5)
+    q = f.q().cpu().numpy().astype(np.float64)
+    r = f.r().cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-5 * np.sqrt(m))
+    np.testing.assert_allclose(q @ r, a, atol=1e-5 * np.sqrt(m) * np.abs(a).max())
+
+
+@pytest.mark.parametrize("m,n,nrhs", [(400, 100, 1), (2000, 64, 3), (128, 128, 2)])
+def test_lstsq_f32_matches_oracle(m, n, nrhs):
+    a = R.normal(3 * m + n, (m, n)).astype(np.float32)
+    b = R.normal(5 * m + n, (m, nrhs) if nrhs > 1 else (m,)).astype(np.float32)
+    x = nl.lstsq(a, b).cpu().numpy().astype(np.float64)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    cols = [b64] if nrhs == 1 else [b64[:, j] for j in range(nrhs)]
+    xs = [x] if nrhs == 1 else [x[:, j] for j in range(nrhs)]
+    for bj, xj in zip(cols, xs):
+        xo = oracle.gels(a64, bj)
+        assert np.max(np.abs(xj - xo)) <= 1e-4 * max(1.0, np.abs(xo).max())
+        rr = bj - a64 @ xj
+        assert np.max(np.abs(a64.T @ rr)) <= 1e-3 * np.linalg.norm(a64) * np.linalg.norm(bj)
+    # QR.solve and apply_q_transpose at float32 agree with lstsq
+    f = nl.qr_factor(a)
+    np.testing.assert_allclose(f.solve(b).cpu().numpy(), x, rtol=0, atol=1e-5 * max(1.0, np.abs(x).max()))
+    qtb = f.apply_q_transpose(b).cpu().numpy().astype(np.float64)
+    q = f.q().cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(qtb[:n], q.T @ b64, atol=1e-4 * np.abs(b64).max() * np.sqrt(m))
+
+
+def test_laswp_f32_is_exact():
+    n, k = 777, 5
+    rng = np.random.default_rng(3)
+    ipiv = np.array([rng.integers(j, n) for j in range(n)], dtype=np.int32)
+    b = rng.standard_normal((n, k)).astype(np.float32)
+    got = nl.laswp(b, ipiv).cpu().numpy()
+    assert got.dtype == np.float32
+    np.testing.assert_array_equal(got, oracle.laswp(b.astype(np.float64), ipiv).astype(np.float32))

@@ -316,3 +316,66 @@ def test_backward_error_closed_form():
     x = np.array([1.0, 1.0])
     b = np.array([2.0, 4.5])  # residual 0.5 on row 2
     assert abs(oracle.backward_error(a, x, b) - 0.5 / (4.0 * 1.0)) < 1e-16
+
+
+def test_batch_entry_points_equal_the_single_problem_functions():
+    # the CPU baselines of the batched configurations run these: one problem
+    # per thread, each exactly the single-problem function
+    from numax_amd import random as R
+    rng = np.random.default_rng(4)
+    sizes = [int(s) for s in rng.integers(1, 60, size=30)]
+    mats, rhss = [], []
+    kinds = (np.arange(30) % 2).astype(np.uint8)
+    for i, n in enumerate(sizes):
+        if kinds[i] == 0:
+            mats.append(rng.uniform(-1, 1, (n, n)) + 0.1 * n * np.eye(n))
+        else:
+            g = rng.standard_normal((n, n))
+            mats.append(g @ g.T / n + np.eye(n))
+        rhss.append(rng.standard_normal(n))
+    n = np.array(sizes, dtype=np.int32)
+    oa = np.zeros(30, dtype=np.int64)
+    ob = np.zeros(30, dtype=np.int64)
+    oa[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    ob[1:] = np.cumsum(n)[:-1]
+    oracle.set_threads(4)
+    x = oracle.array_gesv_batch(np.concatenate([m.ravel() for m in mats]), np.concatenate(rhss), n, kinds, oa, ob)
+    for p in range(30):
+        if kinds[p] == 0:
+            lu, perm, _ = oracle.array_lu_factor(mats[p])
+            xo = oracle.array_lu_solve(lu, perm, rhss[p])
+        else:
+            xo = oracle.array_cholesky_solve(oracle.array_cholesky(mats[p]), rhss[p])
+        assert np.array_equal(x[ob[p]:ob[p] + sizes[p]], xo)
+    a2, n2, oa2, ow2 = R.eigh_batch(16, 3, sizes=(5, 9))
+    w, v = oracle.array_eigh_batch(a2.astype(np.float64), n2, oa2, ow2)
+    for p in range(16):
+        k = int(n2[p])
+        wo, vo = oracle.array_eigh(a2[oa2[p]:oa2[p] + k * k].reshape(k, k).astype(np.float64))
+        assert np.array_equal(w[ow2[p]:ow2[p] + k], wo)
+        assert np.array_equal(v[oa2[p]:oa2[p] + k * k].reshape(k, k), vo)
+    A, B = R.lstsq_batch(6, 40, 5, 2)
+    X = oracle.array_lstsq_batch(A, B)
+    for p in range(6):
+        assert np.array_equal(X[p], oracle.array_lstsq(A[p].astype(np.float64), B[p].astype(np.float64)))
+    oracle.set_threads(1)
+
+
+def test_register_blocked_gemm_matches_numpy_and_is_thread_invariant():
+    # c - a b (and a b^T) against numpy, with ragged sizes around the 4 x 8
+    # register blocks; the result may not depend on the thread count
+    rng = np.random.default_rng(5)
+    for (m, n, k, tb) in [(1, 1, 1, 0), (4, 8, 3, 0), (5, 9, 17, 1), (37, 70, 33, 0), (64, 64, 64, 1)]:
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((n, k) if tb else (k, n))
+        c = rng.standard_normal((m, n))
+        ref = c - a @ (b.T if tb else b)
+        outs = []
+        for t in (1, 3):
+            oracle.set_threads(t)
+            cc = c.copy()
+            oracle._lib.orc_gemm_sub(m, n, k, oracle._p(a), k, oracle._p(b), b.shape[1], tb, oracle._p(cc), n)
+            outs.append(cc)
+        np.testing.assert_allclose(outs[0], ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()) * k)
+        assert np.array_equal(outs[0], outs[1])
+    oracle.set_threads(1)
