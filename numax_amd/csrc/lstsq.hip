@@ -130,6 +130,151 @@ __global__ void __launch_bounds__(256) gels_tsqr_kernel(int count, int m, int n,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Flat Householder for m <= 1024: one 256-thread workgroup per problem, the
+// whole m x 33 [A | b] in registers (thread t holds rows t, t+256, t+512,
+// t+768), one reflector per column over all m rows -- 32 steps per problem
+// instead of the streaming form's 32 per 32 rows.  Per step: sigma and the
+// 33 - K column dot products reduced in the wavefront (recursive halving)
+// and across the four wavefronts through LDS (one barrier each), alpha from
+// the row-K thread through LDS; the update is four rows per thread.  The
+// reflectors are geqr2_panel's (the same dlarfg convention as above), so R
+// and Q^T b are numax's up to rounding.  Rows are loaded coalesced (eight
+// rows per wavefront instruction) and transposed through LDS.
+// ---------------------------------------------------------------------------
+constexpr int kFlatT = 256;               // threads per problem
+constexpr int kFlatR = 4;                 // rows per thread
+constexpr int kFlatMaxM = kFlatT * kFlatR;
+constexpr int kFlatWaves = kFlatT / 64;
+
+struct FlatLds {
+  float stage[64][kCols + 3];    // 64 rows of [A | b] per load round (stride 36: 16-byte aligned)
+  // per step (double-buffered): each wavefront's column sums (sigma last),
+  // then row K's own values
+  float part[2][kFlatWaves + 1][kCols];
+  float alpha[2];
+};
+
+template <int K>
+__device__ __forceinline__ void flat_step(float (&r)[kFlatR][kCols], int t, int n, FlatLds& L) {
+  if (K >= n) return;  // uniform
+  constexpr int NJ = kCols - 1 - K;    // columns to the right, b included
+  constexpr int P2 = pow2_at_least(NJ);
+  constexpr int SPREAD = 64 / P2;
+  const int lane = t & 63, wave = t >> 6;
+  constexpr int buf = K & 1;
+  // with x the column below K: sigma = x^T x and u_j = x^T A[:, j] now; the
+  // reflector's v is 1 on row K and x * scale below, so v^T A[:, j] =
+  // A[K][j] + scale * u_j once alpha gives scale
+  float sg = 0.f;
+  float p[P2];
+#pragma unroll
+  for (int q = 0; q < P2; q++) p[q] = 0.f;
+#pragma unroll
+  for (int i = 0; i < kFlatR; i++) {
+    const int row = t + kFlatT * i;
+    const float x = row > K ? r[i][K] : 0.f;
+    sg += x * x;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) p[j] += x * r[i][K + 1 + j];
+    if (row == K) {
+      L.alpha[buf] = r[i][K];
+#pragma unroll
+      for (int j = 0; j < NJ; j++) L.part[buf][kFlatWaves][j] = r[i][K + 1 + j];  // row K's A[K][j]
+    }
+  }
+  const float tot = halving_sum<P2>(p, lane);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) sg += __shfl_xor(sg, m);
+  if (lane % SPREAD == 0 && lane / SPREAD < NJ) L.part[buf][wave][lane / SPREAD] = tot;
+  if (lane == 0) L.part[buf][wave][kCols - 1] = sg;
+  __syncthreads();
+  float sigma = 0.f;
+#pragma unroll
+  for (int w = 0; w < kFlatWaves; w++) sigma += L.part[buf][w][kCols - 1];
+  const float alpha = L.alpha[buf];
+  if (sigma == 0.f) return;  // uniform: tau = 0, column K already triangular
+  const float beta = -copysignf(sqrtf(alpha * alpha + sigma), alpha);
+  const float tau = (beta - alpha) / beta;
+  const float scale = 1.f / (alpha - beta);
+  float f[NJ];  // tau * v^T A[:, j]
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    float u = 0.f;
+#pragma unroll
+    for (int w = 0; w < kFlatWaves; w++) u += L.part[buf][w][j];
+    f[j] = tau * (L.part[buf][kFlatWaves][j] + scale * u);
+  }
+#pragma unroll
+  for (int i = 0; i < kFlatR; i++) {
+    const int row = t + kFlatT * i;
+    if (row == K) {
+      r[i][K] = beta;
+#pragma unroll
+      for (int j = 0; j < NJ; j++) r[i][K + 1 + j] -= f[j];
+    } else if (row > K) {
+      const float v = r[i][K] * scale;
+#pragma unroll
+      for (int j = 0; j < NJ; j++) r[i][K + 1 + j] -= f[j] * v;
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void flat_all(float (&r)[kFlatR][kCols], int t, int n, FlatLds& L) {
+  if constexpr (K < 32) {
+    flat_step<K>(r, t, n, L);
+    flat_all<K + 1>(r, t, n, L);
+  }
+}
+
+__global__ void __launch_bounds__(kFlatT) gels_flat_kernel(int count, int m, int n, const float* __restrict__ A,
+                                                           const float* __restrict__ B, float* __restrict__ X) {
+  __shared__ FlatLds L;
+  const int t = threadIdx.x;
+  for (int p = blockIdx.x; p < count; p += gridDim.x) {
+    const float* Ap = A + (int64_t)p * m * n;
+    const float* Bp = B + (int64_t)p * m;
+    float r[kFlatR][kCols];
+    // 64 rows per round: thread t loads element t % 64 .. of rows (t / 64) + 4 k
+    // -- consecutive threads read consecutive elements of the packed problem
+#pragma unroll
+    for (int i = 0; i < kFlatR; i++) {
+#pragma unroll
+      for (int round = 0; round < 4; round++) {
+        const int r0 = i * kFlatT + round * 64;  // first row of this round
+        __syncthreads();
+        for (int e = t; e < 64 * n; e += kFlatT) {
+          const int rr = e / n, cc = e % n;
+          L.stage[rr][cc] = r0 + rr < m ? Ap[(int64_t)(r0 + rr) * n + cc] : 0.f;
+        }
+        if (t < 64) L.stage[t][32] = r0 + t < m ? Bp[r0 + t] : 0.f;
+        __syncthreads();
+        if ((t >> 6) == round) {
+#pragma unroll
+          for (int c = 0; c < kCols; c++) r[i][c] = c < n || c == 32 ? L.stage[t & 63][c] : 0.f;
+        }
+      }
+    }
+    __syncthreads();
+    flat_all<0>(r, t, n, L);
+    // R x = (Q^T b)[0:n] in wavefront 0 (rows 0..31 are its lanes' first rows)
+    if (t < 64) {
+      float acc = t < n ? r[0][32] : 0.f;
+#pragma unroll
+      for (int c = 31; c >= 0; c--) {
+        if (c < n) {  // uniform
+          const float xc = readlane_f(acc, c) / readlane_f(r[0][c], c);
+          if (t == c) acc = xc;
+          else if (t < c) acc -= r[0][c] * xc;
+        }
+      }
+      if (t < n) X[(int64_t)p * n + t] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 NMX_API int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const float* A, const float* B, float* X) {
@@ -142,11 +287,18 @@ NMX_API int nmx_batched_gels_f32(nmx_ctx* ctx, int count, int m, int n, const fl
   if (!B) return -6;
   if (!X) return -7;
   if ((int64_t)m * n * 4 >= 0x7FFFFFFF) return NMX_ERR_UNSUPPORTED;
-  const int waves = 4;
-  int grid = ceil_div(count, waves);
-  if (grid > ctx->num_cus * 8) grid = ctx->num_cus * 8;
   ProfScope ps(ctx, K_TSQR, 0.0, 4.0 * (double)count * ((double)m * n + m + n));
-  gels_tsqr_kernel<<<grid, 64 * waves, 0, ctx->stream>>>(count, m, n, A, B, X);
+  if (m <= kFlatMaxM) {
+    // one workgroup per problem, the whole problem in registers
+    const int grid = std::min(count, ctx->num_cus * 16);
+    gels_flat_kernel<<<grid, kFlatT, 0, ctx->stream>>>(count, m, n, A, B, X);
+  } else {
+    // taller problems stream through one wavefront, 32 rows at a time
+    const int waves = 4;
+    int grid = ceil_div(count, waves);
+    if (grid > ctx->num_cus * 8) grid = ctx->num_cus * 8;
+    gels_tsqr_kernel<<<grid, 64 * waves, 0, ctx->stream>>>(count, m, n, A, B, X);
+  }
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
