@@ -84,6 +84,23 @@ int nmx_getrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* LU, int lda, const
 int nmx_laswp_f64(nmx_ctx* ctx, int n, int nrhs, double* B, int ldb, const int32_t* ipiv);
 int nmx_laswp_f32(nmx_ctx* ctx, int n, int nrhs, float* B, int ldb, const int32_t* ipiv);
 
+/* inverse (numax/linalg/basic.mojo:149-195) after lu_factor: A^-1 written to
+ * Ainv (n x n, ld ldi) by LU.solve against the identity -- n right-hand
+ * sides, blocked substitutions.  A singular factor yields non-finite
+ * entries, not an error. */
+int nmx_getri_f64(nmx_ctx* ctx, int n, const double* LU, int lda, const int32_t* ipiv, double* Ainv, int ldi);
+int nmx_getri_f32(nmx_ctx* ctx, int n, const float* LU, int lda, const int32_t* ipiv, float* Ainv, int ldi);
+
+/* lu -> PLU (numax/linalg/lu.mojo:849-975): lu_factor's packed factor and
+ * interchanges unpacked into A = P L U -- L unit lower, U upper, perm[i] the
+ * row of A that row i of L U came from (the interchanges replayed in order
+ * on the identity), P[perm[c]][c] = 1.  Every output is n x n row-major
+ * (perm n entries), device memory. */
+int nmx_lu_unpack_f64(nmx_ctx* ctx, int n, const double* LU, int lda, const int32_t* ipiv, int32_t* perm, double* P,
+                      int ldp, double* L, int ldl, double* U, int ldu);
+int nmx_lu_unpack_f32(nmx_ctx* ctx, int n, const float* LU, int lda, const int32_t* ipiv, int32_t* perm, float* P,
+                      int ldp, float* L, int ldl, float* U, int ldu);
+
 /* LU._diagonal (lu.mojo:355-379): gather diag(A) into an n-vector for
  * det / slogdet. */
 int nmx_diag_f64(nmx_ctx* ctx, int n, const double* A, int lda, double* diag);
@@ -242,6 +259,55 @@ int nmx_batch_autotune(nmx_ctx* ctx, int samples, nmx_batch_cfg* cfg);
  * matrix or eigenvalues would fall outside those sizes, are skipped. */
 int nmx_batched_syevj_f32(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offW,
                           const float* A, int64_t a_elems, float* W, int64_t w_elems, float* V, int sweeps);
+
+/* ---- register-tier operations over batches of small matrices --------------
+ * numax's Array-tier functions (numax/linalg/_array/) for `count` row-major
+ * FP64 matrices of order n[p] in 1..64 (numax's register tier stops at 64,
+ * numax/linalg/common.mojo:30) at element offset offA[p] of A (a_elems
+ * elements), one wavefront per matrix.  Loop orders are numax's and no
+ * multiply-add is fused, so results equal numax's float64 CPU results bit
+ * for bit (slogdet's logarithms to rounding).  A problem whose n is out of
+ * range, or whose matrix or vectors would fall outside the arrays, is
+ * skipped and its outputs left untouched.
+ *   lu_nopiv     lu (_array/lu.mojo:25-55): unpivoted Doolittle, packed L\U
+ *                to LU at offA[p] (LU may be A).
+ *   solve_nopiv  solve (_array/basic.mojo:20-31): unpivoted LU, forward and
+ *                back substitution; b_p at offB[p] of B (b_elems), x_p at
+ *                offB[p] of X.
+ *   inverse      inverse (_array/basic.mojo:34-60): unpivoted, column by
+ *                column of the identity, to Ainv at offA[p].
+ *   det          det (_array/lu.mojo:219-233; pivoted = 0, the unpivoted
+ *                product) or PivotedLU.det (:112-123; pivoted = 1, sign of
+ *                the row swaps times U's diagonal), to det[p].
+ *   slogdet      slogdet / PivotedLU.slogdet (_array/lu.mojo:125-162,
+ *                :236-258): sign[p] in {-1, 0, +1}, logabsdet[p] (-inf when
+ *                U has a zero on its diagonal).
+ *   slogdet_cholesky  (_array/cholesky.mojo:95-116) 2 sum ln L_ii of the
+ *                Cholesky factors L_p (row-major, lower triangle read) at
+ *                offL[p], to logdet[p]. */
+int nmx_batched_lu_nopiv_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const double* A,
+                             int64_t a_elems, double* LU);
+int nmx_batched_solve_nopiv_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
+                                const double* A, int64_t a_elems, const double* B, int64_t b_elems, double* X);
+int nmx_batched_inverse_nopiv_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const double* A,
+                                  int64_t a_elems, double* Ainv);
+int nmx_batched_det_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const double* A,
+                        int64_t a_elems, int pivoted, double* det);
+int nmx_batched_slogdet_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const double* A,
+                            int64_t a_elems, double* sign, double* logabsdet);
+int nmx_batched_slogdet_cholesky_f64(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offL, const double* L,
+                                     int64_t l_elems, double* logdet);
+
+/* map_blocks' structure-of-arrays batch (numax/core/functional.mojo:691-834,
+ * layout :729-735): xs is (n*n + n, batch) row-major -- row j < n*n holds
+ * entry j of every problem's row-major A, row n*n + i entry i of every b --
+ * and ys (n, batch) receives every x, one problem per column, as the
+ * batched-solve step bodies compute it (examples/advanced/batched_solve.mojo
+ * :87-105): kind 0 lu_factor + PivotedLU.solve, 1 cholesky + cholesky_solve
+ * (lower triangle read), 2 unpivoted solve.  n in 1..64; up to n = 8 one GPU
+ * thread holds one problem in registers, as map_blocks' lane does. */
+int nmx_map_blocks_solve_f64(nmx_ctx* ctx, int kind, int n, int64_t batch, const double* xs, double* ys);
+int nmx_map_blocks_solve_f32(nmx_ctx* ctx, int kind, int n, int64_t batch, const float* xs, float* ys);
 
 /* ---- batched tall-skinny least squares ---------------------------------------
  * lstsq(a, b) with method "qr" (numax/linalg/qr.mojo:746-821: Householder

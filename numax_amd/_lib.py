@@ -75,6 +75,21 @@ SIGNATURES = {
     "nmx_batched_syevj_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
                                        _c_int]),
     "nmx_batched_gels_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_vp]),
+    "nmx_getri_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int]),
+    "nmx_getri_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int]),
+    "nmx_lu_unpack_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_int, _c_vp,
+                                   _c_int]),
+    "nmx_lu_unpack_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_vp, _c_int, _c_vp,
+                                   _c_int]),
+    "nmx_batched_lu_nopiv_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
+    "nmx_batched_solve_nopiv_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,
+                                             _c_vp]),
+    "nmx_batched_inverse_nopiv_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
+    "nmx_batched_det_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_int, _c_vp]),
+    "nmx_batched_slogdet_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_vp]),
+    "nmx_batched_slogdet_cholesky_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
+    "nmx_map_blocks_solve_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
+    "nmx_map_blocks_solve_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
     "nmx_batched_gesv": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
                                   _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_batched_gesv_noalloc": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,

@@ -190,6 +190,87 @@ NMX_API int nmx_getrs_f32(nmx_ctx* ctx, int n, int nrhs, const float* LU, int ld
 }
 
 template <typename T>
+__global__ void eye_kernel(int n, T* A, int lda) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  A[(int64_t)i * lda + j] = i == j ? T(1) : T(0);
+}
+
+// inverse (basic.mojo:149-195) after lu_factor: LU.solve against the whole
+// identity, n right-hand sides, so the substitutions are blocked TRSMs with
+// MFMA updates between the diagonal blocks.
+template <typename T>
+static int getri_api(nmx_ctx* ctx, int n, const T* LU, int lda, const int32_t* ipiv, T* Ainv, int ldi) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  if (lda < (n > 1 ? n : 1)) return -4;
+  if (ldi < (n > 1 ? n : 1)) return -7;
+  if (n == 0) return NMX_OK;
+  if (!LU) return -3;
+  if (!ipiv) return -5;
+  if (!Ainv) return -6;
+  eye_kernel<T><<<(unsigned)(((int64_t)n * n + 255) / 256), 256, 0, ctx->stream>>>(n, Ainv, ldi);
+  NMX_HIP(ctx, hipGetLastError());
+  return getrs_api<T>(ctx, n, n, LU, lda, ipiv, Ainv, ldi);
+}
+NMX_API int nmx_getri_f64(nmx_ctx* ctx, int n, const double* LU, int lda, const int32_t* ipiv, double* Ainv,
+                          int ldi) {
+  return getri_api<double>(ctx, n, LU, lda, ipiv, Ainv, ldi);
+}
+NMX_API int nmx_getri_f32(nmx_ctx* ctx, int n, const float* LU, int lda, const int32_t* ipiv, float* Ainv,
+                          int ldi) {
+  return getri_api<float>(ctx, n, LU, lda, ipiv, Ainv, ldi);
+}
+
+// A = P L U from lu_factor's packed factor (lu, lu.mojo:849-975): L (unit
+// diagonal) and U split out, perm[i] = the row of A that row i of L U came
+// from (the interchanges replayed in order on the identity), P[perm[c]][c]
+// = 1.
+template <typename T>
+__global__ void lu_unpack_kernel(int n, const T* __restrict__ LU, int lda, const int32_t* __restrict__ perm,
+                                 T* __restrict__ P, int ldp, T* __restrict__ L, int ldl, T* __restrict__ U, int ldu) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const T v = LU[(int64_t)i * lda + j];
+  L[(int64_t)i * ldl + j] = i > j ? v : (i == j ? T(1) : T(0));
+  U[(int64_t)i * ldu + j] = i <= j ? v : T(0);
+  P[(int64_t)i * ldp + j] = perm[j] == i ? T(1) : T(0);
+}
+template <typename T>
+static int lu_unpack_api(nmx_ctx* ctx, int n, const T* LU, int lda, const int32_t* ipiv, int32_t* perm, T* P,
+                         int ldp, T* L, int ldl, T* U, int ldu) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  const int ld1 = n > 1 ? n : 1;
+  if (lda < ld1) return -4;
+  if (ldp < ld1) return -8;
+  if (ldl < ld1) return -10;
+  if (ldu < ld1) return -12;
+  if (n == 0) return NMX_OK;
+  if (!LU) return -3;
+  if (!ipiv) return -5;
+  if (!perm) return -6;
+  if (!P) return -7;
+  if (!L) return -9;
+  if (!U) return -11;
+  NMX_TRY(perm_from_ipiv(ctx, n, ipiv, perm));
+  lu_unpack_kernel<T><<<(unsigned)(((int64_t)n * n + 255) / 256), 256, 0, ctx->stream>>>(n, LU, lda, perm, P, ldp,
+                                                                                          L, ldl, U, ldu);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+NMX_API int nmx_lu_unpack_f64(nmx_ctx* ctx, int n, const double* LU, int lda, const int32_t* ipiv, int32_t* perm,
+                              double* P, int ldp, double* L, int ldl, double* U, int ldu) {
+  return lu_unpack_api<double>(ctx, n, LU, lda, ipiv, perm, P, ldp, L, ldl, U, ldu);
+}
+NMX_API int nmx_lu_unpack_f32(nmx_ctx* ctx, int n, const float* LU, int lda, const int32_t* ipiv, int32_t* perm,
+                              float* P, int ldp, float* L, int ldl, float* U, int ldu) {
+  return lu_unpack_api<float>(ctx, n, LU, lda, ipiv, perm, P, ldp, L, ldl, U, ldu);
+}
+
+template <typename T>
 static int laswp_api(nmx_ctx* ctx, int n, int nrhs, T* B, int ldb, const int32_t* ipiv) {
   if (!ctx) return -1;
   if (n < 0) return -2;

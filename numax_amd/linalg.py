@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["LU", "NumaxError", "lu_factor", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
+__all__ = ["LU", "PLU", "NumaxError", "lu_factor", "lu", "inverse", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
            "gemm", "PRECISIONS"]
 
 # trailing-update precisions of the mixed-precision solver (nmx_ir_cfg.precision)
@@ -174,6 +174,50 @@ def lu_factor(a, block: int = 16, base: int = 16) -> LU:
         if p != j:
             sign = -sign
     return LU(work, piv[:n], sign, int(info.item()))
+
+
+class PLU:
+    """What numax's ``lu`` returns (lu.mojo:849-874): ``p``, ``l``, ``u`` with
+    ``A = P @ L @ U``, SciPy's ``(p, l, u)``; ``perm[i]`` is the row of ``A``
+    that row ``i`` of ``L @ U`` came from."""
+
+    def __init__(self, p: torch.Tensor, l: torch.Tensor, u: torch.Tensor, perm: torch.Tensor):
+        self.p = p
+        self.l = l
+        self.u = u
+        self.perm = perm
+
+    def __iter__(self):
+        return iter((self.p, self.l, self.u))
+
+
+def lu(a, block: int = 16) -> PLU:
+    """numax ``lu`` (lu.mojo:877-975): ``lu_factor``, then the packed factor
+    and interchanges unpacked on the device into ``P``, ``L``, ``U``."""
+    f = lu_factor(a, block=block)
+    n = f.n
+    dev, dt = f.factored.device, f.factored.dtype
+    p = torch.empty((n, n), dtype=dt, device=dev)
+    l = torch.empty_like(p)
+    u = torch.empty_like(p)
+    perm = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    ld = max(n, 1)
+    _lib.check(_fn("lu_unpack", dt)(_ctx(f.factored).handle, n, _ptr(f.factored), ld, _ptr(f.pivots), _ptr(perm),
+                                     _ptr(p), ld, _ptr(l), ld, _ptr(u), ld), "lu")
+    return PLU(p, l, u, perm[:n])
+
+
+def inverse(a, block: int = 16) -> torch.Tensor:
+    """numax ``inverse`` (basic.mojo:149-195): one ``lu_factor`` and one
+    ``LU.solve`` against the whole identity.  A singular ``a`` does not raise;
+    it yields non-finite entries."""
+    f = lu_factor(a, block=block)
+    n = f.n
+    out = torch.empty((n, n), dtype=f.factored.dtype, device=f.factored.device)
+    ld = max(n, 1)
+    _lib.check(_fn("getri", f.factored.dtype)(_ctx(f.factored).handle, n, _ptr(f.factored), ld, _ptr(f.pivots),
+                                               _ptr(out), ld), "inverse")
+    return out
 
 
 def laswp(b, pivots) -> torch.Tensor:
@@ -608,3 +652,100 @@ def lstsq_batched(a, b) -> torch.Tensor:
     X = torch.zeros((count, n), dtype=torch.float32, device=A.device)
     _lib.check(_lib.load().nmx_batched_gels_f32(_ctx(A).handle, count, m, n, _ptr(A), _ptr(B), _ptr(X)), "lstsq")
     return X
+
+
+# ---- register-tier operations over batches of small matrices ---------------
+
+def _small_batch(a, n, offsets_a):
+    A = _device_tensor(a, torch.float64)
+    count = len(n)
+    N = _device_tensor(np.asarray(n, dtype=np.int32))
+    OA = _device_tensor(np.asarray(offsets_a, dtype=np.int64))
+    return A, count, N, OA
+
+
+def batched_lu_nopiv(a, n, offsets_a) -> torch.Tensor:
+    """numax's Array-tier ``lu`` (_array/lu.mojo:25-55) over a batch of float64
+    matrices of order 1..64 packed back to back (``offsets_a``: element offset
+    of each): unpivoted Doolittle, packed L\\U at the same offsets."""
+    A, count, N, OA = _small_batch(a, n, offsets_a)
+    out = A.clone()
+    _lib.check(_lib.load().nmx_batched_lu_nopiv_f64(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(A), A.numel(),
+                                                     _ptr(out)), "lu")
+    return out
+
+
+def batched_solve_nopiv(a, b, n, offsets_a, offsets_b) -> torch.Tensor:
+    """numax's Array-tier ``solve`` (_array/basic.mojo:20-31): unpivoted LU and
+    two substitutions per problem; ``x_p`` at ``offsets_b[p]`` of the result."""
+    A, count, N, OA = _small_batch(a, n, offsets_a)
+    B = _device_tensor(b, torch.float64)
+    OB = _device_tensor(np.asarray(offsets_b, dtype=np.int64))
+    X = torch.zeros_like(B)
+    _lib.check(_lib.load().nmx_batched_solve_nopiv_f64(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(OB), _ptr(A),
+                                                        A.numel(), _ptr(B), B.numel(), _ptr(X)), "solve")
+    return X
+
+
+def batched_inverse(a, n, offsets_a) -> torch.Tensor:
+    """numax's Array-tier ``inverse`` (_array/basic.mojo:34-60), unpivoted,
+    each inverse at its matrix's offset."""
+    A, count, N, OA = _small_batch(a, n, offsets_a)
+    out = torch.zeros_like(A)
+    _lib.check(_lib.load().nmx_batched_inverse_nopiv_f64(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(A),
+                                                          A.numel(), _ptr(out)), "inverse")
+    return out
+
+
+def batched_det(a, n, offsets_a, pivoted: bool = False) -> torch.Tensor:
+    """numax's Array-tier ``det`` (_array/lu.mojo:219-233, unpivoted) or, with
+    ``pivoted``, ``lu_factor(a).det()`` (:112-123), one value per matrix."""
+    A, count, N, OA = _small_batch(a, n, offsets_a)
+    out = torch.zeros(max(count, 1), dtype=torch.float64, device=A.device)
+    _lib.check(_lib.load().nmx_batched_det_f64(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(A), A.numel(),
+                                                int(bool(pivoted)), _ptr(out)), "det")
+    return out[:count]
+
+
+def batched_slogdet(a, n, offsets_a):
+    """numax's Array-tier ``slogdet`` (_array/lu.mojo:236-258, through
+    ``PivotedLU.slogdet``, :125-162): ``(sign, logabsdet)`` per matrix, sign 0
+    and -inf for an exactly singular factor."""
+    A, count, N, OA = _small_batch(a, n, offsets_a)
+    sign = torch.zeros(max(count, 1), dtype=torch.float64, device=A.device)
+    logabs = torch.zeros_like(sign)
+    _lib.check(_lib.load().nmx_batched_slogdet_f64(_ctx(A).handle, count, _ptr(N), _ptr(OA), _ptr(A), A.numel(),
+                                                    _ptr(sign), _ptr(logabs)), "slogdet")
+    return sign[:count], logabs[:count]
+
+
+def batched_slogdet_cholesky(lower, n, offsets_l) -> torch.Tensor:
+    """numax's ``slogdet_cholesky`` (_array/cholesky.mojo:95-116) of a batch of
+    Cholesky factors: ``2 sum ln L_ii`` per factor."""
+    L, count, N, OL = _small_batch(lower, n, offsets_l)
+    out = torch.zeros(max(count, 1), dtype=torch.float64, device=L.device)
+    _lib.check(_lib.load().nmx_batched_slogdet_cholesky_f64(_ctx(L).handle, count, _ptr(N), _ptr(OL), _ptr(L),
+                                                             L.numel(), _ptr(out)), "slogdet_cholesky")
+    return out[:count]
+
+
+MAP_BLOCKS_KINDS = {"lu": 0, "cholesky": 1, "nopivot": 2}
+
+
+def map_blocks_solve(xs, n: int, kind: str = "lu") -> torch.Tensor:
+    """A map_blocks batched solve (numax/core/functional.mojo:691-834) over its
+    structure-of-arrays layout: ``xs`` is ``(n*n + n, batch)`` -- rows
+    ``0 .. n*n-1`` hold every problem's row-major ``A``, rows ``n*n ..`` its
+    ``b`` -- and the result is ``(n, batch)``.  ``kind`` picks the step body:
+    ``"lu"`` (lu_factor + PivotedLU.solve), ``"cholesky"`` (cholesky +
+    cholesky_solve) or ``"nopivot"`` (Array ``solve``).  float32 or float64."""
+    X = _device_tensor(xs)
+    if kind not in MAP_BLOCKS_KINDS:
+        raise NumaxError(f"map_blocks: kind {kind!r} is not one of {sorted(MAP_BLOCKS_KINDS)}")
+    if X.dim() != 2 or X.shape[0] != n * n + n:
+        raise NumaxError(f"map_blocks: xs must be (n*n + n, batch) = ({n * n + n}, batch), got {tuple(X.shape)}")
+    batch = int(X.shape[1])
+    Y = torch.zeros((n, batch), dtype=X.dtype, device=X.device)
+    _lib.check(_fn("map_blocks_solve", X.dtype)(_ctx(X).handle, MAP_BLOCKS_KINDS[kind], n, batch, _ptr(X), _ptr(Y)),
+               "map_blocks")
+    return Y

@@ -58,6 +58,11 @@ def _load() -> ctypes.CDLL:
         "orc_array_cholesky": (None, [_i, _dp, _dp]),
         "orc_array_cholesky_solve": (None, [_i, _dp, _dp, _dp]),
         "orc_array_solve_nopivot": (None, [_i, _dp, _dp, _dp]),
+        "orc_array_lu_nopivot": (None, [_i, _dp]),
+        "orc_array_inverse_nopivot": (None, [_i, _dp, _dp]),
+        "orc_array_det": (ctypes.c_double, [_i, _dp, _i]),
+        "orc_array_slogdet": (None, [_i, _dp, _dp, _dp]),
+        "orc_array_slogdet_cholesky": (ctypes.c_double, [_i, _dp]),
         "orc_array_eigh": (None, [_i, _dp, _dp, _dp, _i]),
         "orc_array_lstsq": (None, [_i, _i, _dp, _dp, _dp]),
         "orc_backward_error": (ctypes.c_double, [_i, _i, _dp, _i, _dp, _i, _dp, _i]),
@@ -231,6 +236,42 @@ def array_solve(a, b):
     x = np.zeros_like(bb)
     _lib.orc_array_solve_nopivot(a.shape[0], _p(a), _p(bb), _p(x))
     return x
+
+
+def array_lu(a):
+    """Register tier ``lu``: unpivoted Doolittle, packed L\\U."""
+    f = _f64(a).copy()
+    _lib.orc_array_lu_nopivot(f.shape[0], _p(f))
+    return f
+
+
+def array_inverse(a):
+    """Register tier ``inverse`` (unpivoted)."""
+    a = _f64(a)
+    out = np.zeros_like(a)
+    _lib.orc_array_inverse_nopivot(a.shape[0], _p(a), _p(out))
+    return out
+
+
+def array_det(a, pivoted: bool = False) -> float:
+    """Register tier ``det`` (unpivoted) or ``PivotedLU.det``."""
+    a = _f64(a)
+    return float(_lib.orc_array_det(a.shape[0], _p(a), int(pivoted)))
+
+
+def array_slogdet(a) -> tuple[float, float]:
+    """``PivotedLU.slogdet`` of ``lu_factor(a)``: ``(sign, ln|det|)``."""
+    a = _f64(a)
+    s = np.zeros(1)
+    la = np.zeros(1)
+    _lib.orc_array_slogdet(a.shape[0], _p(a), _p(s), _p(la))
+    return float(s[0]), float(la[0])
+
+
+def array_slogdet_cholesky(lower) -> float:
+    """``slogdet_cholesky``: ``2 sum ln L_ii``."""
+    lower = _f64(lower)
+    return float(_lib.orc_array_slogdet_cholesky(lower.shape[0], _p(lower)))
 
 
 def array_eigh(a, sweeps: int = 12):

@@ -698,6 +698,80 @@ void orc_array_solve_nopivot(int n, const double* a, const double* b, double* x)
   free(f); free(y);
 }
 
+/* lu (_array/lu.mojo:25-55): unpivoted Doolittle in place, packed L\U. */
+void orc_array_lu_nopivot(int n, double* f) {
+  for (int k = 0; k < n; k++) {
+    double pivot = guard_nonzero(f[k * n + k], PIVOT_FLOOR);
+    for (int i = k + 1; i < n; i++) {
+      double fac = f[i * n + k] / pivot;
+      f[i * n + k] = fac;
+      for (int j = k + 1; j < n; j++) f[i * n + j] = f[i * n + j] - fac * f[k * n + j];
+    }
+  }
+}
+
+/* inverse (_array/basic.mojo:34-60): unpivoted lu, then forward and back
+ * substitution of each column of the identity. */
+void orc_array_inverse_nopivot(int n, const double* a, double* out) {
+  double* f = (double*)malloc(sizeof(double) * n * n);
+  double* e = (double*)malloc(sizeof(double) * n);
+  double* y = (double*)malloc(sizeof(double) * n);
+  double* x = (double*)malloc(sizeof(double) * n);
+  memcpy(f, a, sizeof(double) * n * n);
+  orc_array_lu_nopivot(n, f);
+  for (int c = 0; c < n; c++) {
+    for (int i = 0; i < n; i++) e[i] = i == c ? 1.0 : 0.0;
+    array_forward(n, f, e, y, 1);
+    array_back(n, f, y, x);
+    for (int i = 0; i < n; i++) out[i * n + c] = x[i];
+  }
+  free(f); free(e); free(y); free(x);
+}
+
+/* det (_array/lu.mojo:219-233, pivoted = 0): the unpivoted lu's diagonal
+ * product; PivotedLU.det (:112-123, pivoted = 1): sign times the pivoted
+ * factor's diagonal product, multiplied in row order. */
+double orc_array_det(int n, const double* a, int pivoted) {
+  double* f = (double*)malloc(sizeof(double) * n * n);
+  int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  memcpy(f, a, sizeof(double) * n * n);
+  double product;
+  if (pivoted) {
+    product = (double)orc_array_lu_factor(n, f, perm);
+  } else {
+    orc_array_lu_nopivot(n, f);
+    product = 1.0;
+  }
+  for (int i = 0; i < n; i++) product = product * f[i * n + i];
+  free(f); free(perm);
+  return product;
+}
+
+/* PivotedLU.slogdet (_array/lu.mojo:125-162) of lu_factor(a): sign 0 and
+ * -inf at the first exact zero on the diagonal. */
+void orc_array_slogdet(int n, const double* a, double* sign_out, double* logabs_out) {
+  double* f = (double*)malloc(sizeof(double) * n * n);
+  int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  memcpy(f, a, sizeof(double) * n * n);
+  double sign = (double)orc_array_lu_factor(n, f, perm), total = 0.0;
+  for (int i = 0; i < n; i++) {
+    double entry = f[i * n + i];
+    if (entry == 0.0) { sign = 0.0; total = log(0.0); break; }
+    if (entry < 0.0) sign = -sign;
+    total += log(fabs(entry));
+  }
+  *sign_out = sign;
+  *logabs_out = total;
+  free(f); free(perm);
+}
+
+/* slogdet_cholesky (_array/cholesky.mojo:95-116): 2 * sum ln(L_ii). */
+double orc_array_slogdet_cholesky(int n, const double* l) {
+  double total = 0.0;
+  for (int i = 0; i < n; i++) total = total + log(l[i * n + i]);
+  return total * 2.0;
+}
+
 /* _jacobi_rotation (_array/eigen.mojo:34-75). */
 static void jacobi_rotation(double num, double den, double* c, double* s) {
   double twice = 2.0 * num;

@@ -87,3 +87,16 @@ def map_blocks_systems(batch: int = 1003, n: int = 4):
     a = np.einsum("pik,pjk->pij", m, m) + 4.0 * np.eye(n)
     b = np.tile(1.0 + 0.25 * np.arange(n), (batch, 1))
     return a, b
+
+# tests/linalg/test_slogdet.mojo:36-48 -- a sign-negative 3x3, and a
+# singular matrix (row 1 = 2 row 0): sign 0, logabsdet -inf.  The test's
+# docstring calls the first determinant -30; its assertions compare with
+# det() instead, and the cofactor expansion 2(3(-2) - 2) + (-2) gives -18.
+SLOGDET_M3 = np.array([[2.0, -1.0, 0.0], [1.0, 3.0, 1.0], [0.0, 2.0, -2.0]])
+SLOGDET_M3_DET = -18.0
+SLOGDET_SINGULAR = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [7.0, 8.0, 10.0]])
+
+# tests/linalg/test_linalg.mojo:65-82, :170-173 -- spd3 (det 67 by cofactors)
+# and general3 (leading minors nonzero, so the unpivoted LU is valid).
+SPD3 = np.array([[4.0, 2.0, 1.0], [2.0, 5.0, 3.0], [1.0, 3.0, 6.0]])
+GENERAL3 = np.array([[2.0, 1.0, -1.0], [-3.0, -1.0, 2.0], [-2.0, 1.0, 2.0]])

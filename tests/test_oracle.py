@@ -379,3 +379,81 @@ def test_register_blocked_gemm_matches_numpy_and_is_thread_invariant():
         np.testing.assert_allclose(outs[0], ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()) * k)
         assert np.array_equal(outs[0], outs[1])
     oracle.set_threads(1)
+
+
+# ------------------------------------------- register tier: lu / inverse / det
+def _dyadic_lu(seed, n):
+    """A = L0 U0 with small integer entries and power-of-two pivots: every
+    step of the unpivoted Doolittle is exact, so it must return L0 \\ U0."""
+    rng = np.random.default_rng(seed)
+    l0 = np.tril(rng.integers(-3, 4, size=(n, n)).astype(np.float64), -1) + np.eye(n)
+    u0 = np.triu(rng.integers(-3, 4, size=(n, n)).astype(np.float64), 1)
+    u0[np.arange(n), np.arange(n)] = 2.0 ** rng.integers(-2, 3, size=n)
+    return l0, u0
+
+
+@pytest.mark.parametrize("seed,n", [(1, 3), (2, 6), (3, 9)])
+def test_array_lu_nopivot_recovers_exact_factors(seed, n):
+    l0, u0 = _dyadic_lu(seed, n)
+    f = oracle.array_lu(l0 @ u0)
+    np.testing.assert_array_equal(np.tril(f, -1) + np.eye(n), l0)
+    np.testing.assert_array_equal(np.triu(f), u0)
+
+
+def test_array_lu_nopivot_zero_pivot_uses_the_floor():
+    # the exchange matrix: the zero pivot is divided as 1e-30 and kept as 0
+    f = oracle.array_lu(kat.EXCHANGE)
+    assert f[0, 0] == 0.0 and f[1, 0] == 1.0 / 1e-30 and f[1, 1] == -(1.0 / 1e-30)
+    # so the unpivoted det is 0 where the pivoted one is -1
+    assert oracle.array_det(kat.EXCHANGE) == 0.0
+    assert oracle.array_det(kat.EXCHANGE, pivoted=True) == kat.EXCHANGE_DET
+
+
+def test_array_det_known_answers():
+    assert abs(oracle.array_det(kat.SPD3) - kat.SPD3_DET) <= 1e-12
+    assert abs(oracle.array_det(kat.SLOGDET_M3, pivoted=True) - kat.SLOGDET_M3_DET) <= 1e-12
+    rng = np.random.default_rng(4)
+    for n in (1, 5, 17, 40):
+        a = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+        want = np.linalg.det(a)
+        assert abs(oracle.array_det(a) - want) <= 1e-12 * abs(want)
+        assert abs(oracle.array_det(a, pivoted=True) - want) <= 1e-12 * abs(want)
+
+
+def test_array_slogdet_known_answers():
+    s, la = oracle.array_slogdet(kat.SLOGDET_M3)
+    assert s == -1.0 and abs(la - np.log(18.0)) <= 1e-12
+    s, la = oracle.array_slogdet(kat.SLOGDET_SINGULAR)
+    assert s == 0.0 and la == -np.inf
+    # test_slogdet.mojo:92-118's diagonal of 40s, at the register tier's 64
+    s, la = oracle.array_slogdet(40.0 * np.eye(64))
+    assert s == 1.0 and abs(la - 64 * np.log(40.0)) <= 1e-12
+    rng = np.random.default_rng(5)
+    for n in (2, 9, 33, 64):
+        a = rng.standard_normal((n, n))
+        ws, wl = np.linalg.slogdet(a)
+        s, la = oracle.array_slogdet(a)
+        assert s == ws and abs(la - wl) <= 1e-12 * max(1.0, abs(wl))
+
+
+def test_array_slogdet_cholesky():
+    l = oracle.array_cholesky(kat.SPD3)
+    assert abs(oracle.array_slogdet_cholesky(l) - np.log(kat.SPD3_DET)) <= 1e-14
+    a = kat.spd_n(30)
+    assert abs(oracle.array_slogdet_cholesky(oracle.array_cholesky(a)) - np.linalg.slogdet(a)[1]) <= 1e-12
+
+
+def test_array_inverse():
+    # test_linalg.mojo's general3 (leading minors nonzero)
+    inv = oracle.array_inverse(kat.GENERAL3)
+    np.testing.assert_allclose(kat.GENERAL3 @ inv, np.eye(3), atol=1e-13)
+    # a diagonal matrix inverts to the rounded reciprocals exactly
+    d = np.array([3.0, -7.0, 0.1, 11.0])
+    np.testing.assert_array_equal(oracle.array_inverse(np.diag(d)), np.diag(1.0 / d))
+    rng = np.random.default_rng(6)
+    for n in (1, 8, 31, 64):
+        a = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+        np.testing.assert_allclose(oracle.array_inverse(a), np.linalg.inv(a), rtol=0, atol=1e-13)
+        # inverse = solve against each column of the identity
+        np.testing.assert_array_equal(oracle.array_inverse(a)[:, n // 2],
+                                      oracle.array_solve(a, np.eye(n)[:, n // 2]))
