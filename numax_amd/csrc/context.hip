@@ -1,8 +1,6 @@
 // Context, workspace, mailbox, status strings and the Philox input generator.
 #include <stdio.h>
 #include <math.h>
-#include <stdlib.h>
-
 #include "nmx_internal.h"
 
 NMX_API int nmx_version(void) { return 1; }
@@ -93,6 +91,17 @@ NMX_API int nmx_ctx_create(int device, void* stream, nmx_ctx** out) {
   return NMX_OK;
 }
 
+int nmx_aux_stream(nmx_ctx* c) {
+  if (!c->aux) {
+    if (c->capturing) return NMX_ERR_UNSUPPORTED;  // creating a stream under capture
+    NMX_HIP(c, hipSetDevice(c->device));
+    NMX_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  }
+  if (!c->ahead_ev) NMX_HIP(c, hipEventCreateWithFlags(&c->ahead_ev, hipEventDisableTiming));
+  if (!c->behind_ev) NMX_HIP(c, hipEventCreateWithFlags(&c->behind_ev, hipEventDisableTiming));
+  return NMX_OK;
+}
+
 // Releases every buffer of the context (the caller made sure no captured
 // graph can still replay kernels that read them).
 static void ctx_release(nmx_ctx* c) {
@@ -101,6 +110,12 @@ static void ctx_release(nmx_ctx* c) {
   for (auto e : c->pool) hipEventDestroy(e);
   for (void* p : c->retired) hipFree(p);
   if (c->switch_ev) hipEventDestroy(c->switch_ev);
+  if (c->aux) {
+    hipStreamSynchronize(c->aux);
+    hipStreamDestroy(c->aux);
+  }
+  if (c->ahead_ev) hipEventDestroy(c->ahead_ev);
+  if (c->behind_ev) hipEventDestroy(c->behind_ev);
   if (c->ws) hipFree(c->ws);
   if (c->mailbox) hipFree(c->mailbox);
   if (c->err) hipFree(c->err);

@@ -1,4 +1,6 @@
 // Blocked LU and Cholesky drivers and their C-ABI entry points.
+#include <type_traits>
+
 #include "nmx_internal.h"
 
 template <typename T>
@@ -18,32 +20,28 @@ namespace {
 // a triangular solve over the panel's own rows and an MFMA GEMM below them.
 // After every leaf its interchanges are replayed over all other columns.
 template <typename T>
-int leaf(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, LeafHook hook, void* user);
-template <>
-int leaf<double>(nmx_ctx* ctx, int n, double* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, LeafHook,
-                 void*) {
-  return getf2_leaf_launch<double>(ctx, n, A, lda, k, k, nb, ipiv, info, 1);
-}
-template <>
-int leaf<float>(nmx_ctx* ctx, int n, float* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, LeafHook hook,
-                void* user) {
-  if (hook) return hook(ctx, user, n, A, lda, k, k, nb, ipiv, info);
-  return getf2_leaf_launch<float>(ctx, n, A, lda, k, k, nb, ipiv, info, 1);
-}
-
-template <typename T>
-int panel_rec(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, int base,
-              LeafHook hook, void* user) {
-  if (nb <= base) return leaf<T>(ctx, n, A, lda, ipiv, info, k, nb, hook, user);
+int panel_rec(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, int base) {
+  if (nb <= base) return getf2_leaf_launch<T>(ctx, n, A, lda, k, k, nb, ipiv, info, 1);
   int half = nb / 2;
   if (half > base) half = (half / base) * base;
-  NMX_TRY(panel_rec<T>(ctx, n, A, lda, ipiv, info, k, half, base, hook, user));
+  NMX_TRY(panel_rec<T>(ctx, n, A, lda, ipiv, info, k, half, base));
   NMX_TRY(trsm_left<T>(ctx, 0, 1, 0, half, nb - half, AT(A, lda, k, k), lda, AT(A, lda, k, k + half), lda));
   const int rows = n - k - half, cols = nb - half;
   if (rows > 0 && cols > 0)
     NMX_TRY(gemm_launch<T>(ctx, 0, 0, rows, cols, half, T(-1), AT(A, lda, k + half, k), lda,
                            AT(A, lda, k, k + half), lda, T(1), AT(A, lda, k + half, k + half), lda, 0));
-  return panel_rec<T>(ctx, n, A, lda, ipiv, info, k + half, nb - half, base, hook, user);
+  return panel_rec<T>(ctx, n, A, lda, ipiv, info, k + half, nb - half, base);
+}
+
+// One panel: the recursive exact-pivoting panel, or (FP32 with a hook) the
+// hook's panel.
+template <typename T>
+int panel(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int k, int nb, int base, PanelHook hook,
+          void* user) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (hook) return hook(ctx, user, n, A, lda, k, nb, ipiv, info);
+  }
+  return panel_rec<T>(ctx, n, A, lda, ipiv, info, k, nb, base);
 }
 
 }  // namespace
@@ -54,13 +52,13 @@ int panel_rec(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, 
 // its FP16 / BF16 MFMA update in here).
 template <typename T>
 int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base,
-                  TrailingUpdate update, void* user, LeafHook leaf_hook) {
+                  TrailingUpdate update, void* user, PanelHook panel_hook) {
   if (block <= 0) block = 256;
   if (base <= 0 || base > kLeafW) base = kLeafW;
   NMX_TRY(zero_info(ctx, info));
   for (int k = 0; k < n; k += block) {
     const int nb = block < n - k ? block : n - k;
-    NMX_TRY(panel_rec<T>(ctx, n, A, lda, ipiv, info, k, nb, base, leaf_hook, user));
+    NMX_TRY(panel<T>(ctx, n, A, lda, ipiv, info, k, nb, base, panel_hook, user));
     const int m = n - k - nb;
     if (m > 0) {
       // U12 := L11^-1 A12 over the whole block row, then A22 -= L21 U12.
@@ -81,7 +79,7 @@ int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info,
   return getrf_impl_ex<T>(ctx, n, A, lda, ipiv, info, block, base, nullptr, nullptr);
 }
 template int getrf_impl_ex<float>(nmx_ctx*, int, float*, int, int32_t*, int32_t*, int, int, TrailingUpdate, void*,
-                                   LeafHook);
+                                   PanelHook);
 
 // _potrf_blocked (numax/linalg/cholesky.mojo:59-230): diagonal block, panel
 // solve X := X L11^-T, and the lower block triangle of A22 -= L21 L21^T.

@@ -946,36 +946,95 @@ static void mp_free(nmx_ctx* ctx, MpFactor& f) {
 
 int zero_info(nmx_ctx* ctx, int32_t* info);
 
+// The operands of step k's update A22 -= L21 U12 (L21 m x nb, U12 nb x m)
+// scaled and converted to FP16 / BF16: L21 as is, U12 transposed (row j of
+// the copy is trailing column j).
 template <int BF, int SPLIT>
-static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
+static int mp_convert(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   typedef typename Lo<BF>::t lt;
-  float* F = f.F;
   const int n = f.n;
-  const float* L21 = F + (int64_t)(k + nb) * n + k;   // m x nb, ld n
-  const float* U12 = F + (int64_t)k * n + k + nb;     // nb x m, ld n
+  const float* L21 = f.F + (int64_t)(k + nb) * n + k;  // m x nb, ld n
+  const float* U12 = f.F + (int64_t)k * n + k + nb;    // nb x m, ld n
   lt* base = (lt*)f.lo_buf;
   const size_t op = (size_t)m * nb;
-  lt* Ah = base;
-  lt* Al = base + op;
-  lt* Bh = base + 2 * op;
-  lt* Bl = base + 3 * op;
   zero_u32k<<<1, 64, 0, ctx->stream>>>(f.maxbits, 2);
   const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
   // both scale words in one launch, then both conversions in one
   const int ga = std::min(m, kAbsmaxBlocks), gb = std::min(nb, kAbsmaxBlocks);
   absmax2_kernel<<<ga + gb, 256, 0, ctx->stream>>>(ga, m, nb, L21, n, f.maxbits, nb, m, U12, n, f.maxbits + 1);
-  convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.maxbits, Ah, Al, U12, n,
-                                                               f.maxbits + 1, Bh, Bl);
+  convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.maxbits, base, base + op, U12, n,
+                                                               f.maxbits + 1, base + 2 * op, base + 3 * op);
   NMX_HIP(ctx, hipGetLastError());
-  const int tiles = ceil_div(m, MT);
-  const int grid = ceil_div(tiles * tiles, kXcds) * kXcds;
-  {
-    ProfScope ps(ctx, K_MP_GEMM, 2.0 * m * (double)m * nb * (SPLIT ? 3 : 1),
-                 8.0 * (double)m * m + (double)sizeof(lt) * 2.0 * m * nb * (SPLIT ? 2 : 1));
-    mp_gemm_kernel<BF, SPLIT><<<grid, kMpThreads, 0, ctx->stream>>>(m, m, nb, Ah, Al, Bh, Bl, f.maxbits, f.maxbits + 1,
-                                                              F + (int64_t)(k + nb) * n + k + nb, n, tiles, tiles);
-    NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+// Trailing columns j0 .. j1 of step k's update, from mp_convert's copies.
+template <int BF, int SPLIT>
+static int mp_gemm_cols(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m, int j0, int j1) {
+  typedef typename Lo<BF>::t lt;
+  const int n = f.n, cols = j1 - j0;
+  if (cols <= 0) return NMX_OK;
+  lt* base = (lt*)f.lo_buf;
+  const size_t op = (size_t)m * nb;
+  const int tm = ceil_div(m, MT), tn = ceil_div(cols, MT);
+  const int grid = ceil_div(tm * tn, kXcds) * kXcds;
+  ProfScope ps(ctx, K_MP_GEMM, 2.0 * m * (double)cols * nb * (SPLIT ? 3 : 1),
+               8.0 * (double)m * cols + (double)sizeof(lt) * (double)(m + cols) * nb * (SPLIT ? 2 : 1));
+  mp_gemm_kernel<BF, SPLIT><<<grid, kMpThreads, 0, ctx->stream>>>(
+      m, cols, nb, base, base + op, base + 2 * op + (size_t)j0 * nb, base + 3 * op + (size_t)j0 * nb, f.maxbits,
+      f.maxbits + 1, f.F + (int64_t)(k + nb) * n + k + nb + j0, n, tm, tn);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+template <int BF, int SPLIT>
+static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
+  NMX_TRY((mp_convert<BF, SPLIT>(ctx, f, k, nb, m)));
+  return mp_gemm_cols<BF, SPLIT>(ctx, f, k, nb, m, 0, m);
+}
+
+// The tournament-pivoted factorization with one panel of look-ahead: after
+// panel k, its U12 and the update of the next panel's columns run on the
+// main stream, the rest of the update -- most of the MFMA work -- on the
+// context's second stream, while the main stream factors panel k+1, a
+// latency-bound run of single-wavefront tournaments that leaves most of the
+// chip to the update.  Panel k+1's interchanges reach the columns right of
+// it only once that update has finished (the conversion buffers and scale
+// words are reused only after that wait too).
+template <int BF, int SPLIT>
+static int mp_getrf_lookahead(nmx_ctx* ctx, MpFactor& f) {
+  const int n = f.n, bw = f.block;
+  float* F = f.F;
+  NMX_TRY(nmx_aux_stream(ctx));
+  NMX_TRY(zero_info(ctx, f.info));
+  bool behind = false;  // an update is queued on the second stream
+  for (int k = 0; k < n; k += bw) {
+    const int nb = std::min(bw, n - k);
+    NMX_TRY(tslu_panel_launch(ctx, f.ts, f.ts_bytes, n, F, n, k, nb, f.ipiv, f.info, 0));
+    NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, 0, k));
+    if (behind) {
+      NMX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->behind_ev, 0));
+      behind = false;
+    }
+    NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, k + nb, n));
+    const int m = n - k - nb;
+    if (m <= 0) break;
+    NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, nb, m, F + (int64_t)k * n + k, n, F + (int64_t)k * n + k + nb, n));
+    NMX_TRY((mp_convert<BF, SPLIT>(ctx, f, k, nb, m)));
+    const int near = std::min(bw, m);
+    NMX_TRY((mp_gemm_cols<BF, SPLIT>(ctx, f, k, nb, m, 0, near)));
+    if (m > near) {
+      NMX_HIP(ctx, hipEventRecord(ctx->ahead_ev, ctx->stream));
+      NMX_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->ahead_ev, 0));
+      {
+        StreamSwap on_aux(ctx, ctx->aux);
+        NMX_TRY((mp_gemm_cols<BF, SPLIT>(ctx, f, k, nb, m, near, m)));
+      }
+      NMX_HIP(ctx, hipEventRecord(ctx->behind_ev, ctx->aux));
+      behind = true;
+    }
   }
+  if (behind) NMX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->behind_ev, 0));
   return NMX_OK;
 }
 
@@ -1011,10 +1070,10 @@ static int mp_update_hook(nmx_ctx* ctx, void* user, int k, int nb, int m) {
   }
 }
 
-static int tslu_hook(nmx_ctx* ctx, void* user, int n, float* A, int lda, int r0, int c0, int w, int32_t* ipiv,
+static int tslu_hook(nmx_ctx* ctx, void* user, int n, float* A, int lda, int k, int nb, int32_t* ipiv,
                      int32_t* info) {
   MpFactor& f = *(MpFactor*)user;
-  return tslu_leaf_launch(ctx, f.ts, f.ts_bytes, n, A, lda, r0, c0, w, ipiv, info);
+  return tslu_panel_launch(ctx, f.ts, f.ts_bytes, n, A, lda, k, nb, ipiv, info, 1);
 }
 
 static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed, int stop_after_transform = 0) {
@@ -1029,8 +1088,16 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
       NMX_HIP(ctx, hipGetLastError());
     }
     if (stop_after_transform) return zero_info(ctx, f.info);
-    NMX_TRY(getrf_impl_ex<float>(ctx, n, f.F, n, f.ipiv, f.info, f.block, kLeafW,
-                                 f.prec == 3 ? nullptr : mp_update_hook, &f, f.pivot == 2 ? tslu_hook : nullptr));
+    if (f.pivot == 2 && f.prec != 3) {
+      switch (f.prec) {
+        case 0: NMX_TRY((mp_getrf_lookahead<0, 0>(ctx, f))); break;
+        case 1: NMX_TRY((mp_getrf_lookahead<1, 0>(ctx, f))); break;
+        default: NMX_TRY((mp_getrf_lookahead<0, 1>(ctx, f))); break;
+      }
+    } else {
+      NMX_TRY(getrf_impl_ex<float>(ctx, n, f.F, n, f.ipiv, f.info, f.block, kLeafW,
+                                   f.prec == 3 ? nullptr : mp_update_hook, &f, f.pivot == 2 ? tslu_hook : nullptr));
+    }
     NMX_TRY(perm_from_ipiv(ctx, n, f.ipiv, f.perm));
     const int nblk = ceil_div(n, TB);
     tri_inv_kernel<0><<<nblk, 256, 0, ctx->stream>>>(n, f.F, n, f.invL);

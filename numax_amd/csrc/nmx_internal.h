@@ -79,6 +79,11 @@ struct nmx_ctx {
   std::vector<void*> retired;
   // orders a newly set stream after the work queued on the previous one
   hipEvent_t switch_ev = nullptr;
+  // second stream (default priority) for work that overlaps the main
+  // stream's within one call, and the two events that order them: created
+  // on first use, joined back into `stream` before the call returns
+  hipStream_t aux = nullptr;
+  hipEvent_t ahead_ev = nullptr, behind_ev = nullptr;
   // this device's copy of the library's static scratch array (the device
   // address hipGetSymbolAddress returned at context creation) and its size:
   // the allocation-free entry points work in it
@@ -113,6 +118,18 @@ struct ProfScope {
   bool on;
   ProfScope(nmx_ctx* c, int kind, double flops, double bytes);
   ~ProfScope();
+};
+
+// The context's second stream and its two events, created on first use.
+int nmx_aux_stream(nmx_ctx* ctx);
+
+// Points the context at another stream for a scope (launch helpers and
+// ProfScope use ctx->stream), then back.
+struct StreamSwap {
+  nmx_ctx* ctx;
+  hipStream_t saved;
+  StreamSwap(nmx_ctx* c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+  ~StreamSwap() { ctx->stream = saved; }
 };
 
 // Workspace of at least `bytes`, 256-byte aligned; grows (with a stream sync)
@@ -163,21 +180,23 @@ int getrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info,
 // Block-level trailing update hook: A22 -= L21 U12 for the step at column k
 // with panel width nb and trailing order m.
 typedef int (*TrailingUpdate)(nmx_ctx* ctx, void* user, int k, int nb, int m);
-// Leaf hook of the recursive panel (FP32 only): factors columns c0..c0+w over
-// rows r0..n, writes ipiv[r0..r0+w) and replays the interchanges over the
-// other columns, like getf2_leaf_launch with swap_others.
-typedef int (*LeafHook)(nmx_ctx* ctx, void* user, int n, float* A, int lda, int r0, int c0, int w, int32_t* ipiv,
-                        int32_t* info);
+// Panel hook (FP32 only): factors the panel columns k..k+nb over rows k..n,
+// writes ipiv[k..k+nb) and applies the interchanges to every column of A,
+// like panel_rec.
+typedef int (*PanelHook)(nmx_ctx* ctx, void* user, int n, float* A, int lda, int k, int nb, int32_t* ipiv,
+                         int32_t* info);
 template <typename T>
 int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* info, int block, int base,
-                  TrailingUpdate update, void* user, LeafHook leaf = nullptr);
+                  TrailingUpdate update, void* user, PanelHook panel = nullptr);
 
-// Tournament-pivoted leaf of the mixed-precision factorization (tslu.hip):
+// Tournament-pivoted panel of the mixed-precision factorization (tslu.hip):
 // scratch of tslu_scratch_bytes(n), cleared once by tslu_scratch_init.
 size_t tslu_scratch_bytes(int n);
 int tslu_scratch_init(nmx_ctx* ctx, void* scratch);
-int tslu_leaf_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int r0, int c0,
-                     int w, int32_t* ipiv, int32_t* info);
+int tslu_panel_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int k, int nb,
+                      int32_t* ipiv, int32_t* info, int swap_outside);
+// The last panel's interchanges over columns c_lo .. c_hi outside it.
+int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi);
 
 template <typename T>
 int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);

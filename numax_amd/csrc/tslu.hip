@@ -20,7 +20,11 @@
 //      that bring the selected rows to the top;
 //   4. the apply kernel writes L11 \ U11 on top and L21 = A21 U11^-1 below,
 //      one row per thread, L11 recomputed the same way from the selected
-//      rows' original values.
+//      rows' original values; the same launch moves the rows in the panel's
+//      other columns and forms U12 = L11^-1 A12 right of the leaf, so one
+//      GEMM (A22 -= L21 U12) finishes the leaf.  The panel is a flat run of
+//      such leaves, and the columns outside it see the interchanges once,
+//      after the panel.
 //
 // Rows are distinct candidates and numbers stay FP32.  What differs from
 // exact partial pivoting is only which rows become pivots: the tournament's
@@ -63,8 +67,17 @@ struct FinalState {
 struct GeppState {
   uint32_t km[kTsR], kc[kTsR];  // key mask / slot code of register row r (0 once retired)
   int sel;                      // lane j: row chosen at step j
+  int selslot;                  // lane j: its slot (register row r, lane l: r * 64 + l)
   bool done;                    // uniform: w columns taken, or no candidate left in this set
 };
+
+template <int J>
+__device__ __forceinline__ uint32_t column_kmax(const float (&v)[kTsR][kTsW], const GeppState& g) {
+  uint32_t k = 0;
+#pragma unroll
+  for (int r = 0; r < kTsR; r++) k = max(k, (__float_as_uint(v[r][J]) & g.km[r]) | g.kc[r]);
+  return wave_umax(k);
+}
 
 // One step of partial pivoting over the wavefront's rows for compile-time
 // column J (so every register index is static): lane `lane`, register row r
@@ -74,10 +87,7 @@ struct GeppState {
 template <int J>
 __device__ __forceinline__ void gepp_step(float (&v)[kTsR][kTsW], const int (&rid)[kTsR], int w, int lane,
                                           float (*rows)[kTsW], int r0, bool final, GeppState& g, FinalState& fs) {
-  uint32_t k = 0;
-#pragma unroll
-  for (int r = 0; r < kTsR; r++) k = max(k, (__float_as_uint(v[r][J]) & g.km[r]) | g.kc[r]);
-  const uint32_t kmax = wave_umax(k);
+  const uint32_t kmax = column_kmax<J>(v, g);
   g.done = g.done || J >= w || kmax == 0;
   if (g.done) return;
   const int slot = 256 - (int)(kmax & 0x1FFu);
@@ -102,7 +112,10 @@ __device__ __forceinline__ void gepp_step(float (&v)[kTsR][kTsW], const int (&ri
     for (int r = 0; r < kTsR; r++)
       if (r == wr) g.km[r] = g.kc[r] = 0u;  // retired
   }
-  if (lane == J) g.sel = prow_id;
+  if (lane == J) {
+    g.sel = prow_id;
+    g.selslot = slot;
+  }
   const float p = su[J];
   const float rinv = __builtin_amdgcn_rcpf(p == 0.0f ? kTsFloor : p);
   float l[kTsR];
@@ -149,7 +162,7 @@ __device__ __forceinline__ void gepp_step(float (&v)[kTsR][kTsW], const int (&ri
 template <int... Js>
 __device__ __forceinline__ int gepp_columns(std::integer_sequence<int, Js...>, float (&v)[kTsR][kTsW],
                                             const int (&rid)[kTsR], int w, int lane, float (*rows)[kTsW], int r0,
-                                            bool final, FinalState& fs) {
+                                            bool final, FinalState& fs, int& selslot) {
   GeppState g;
 #pragma unroll
   for (int r = 0; r < kTsR; r++) {
@@ -158,8 +171,10 @@ __device__ __forceinline__ int gepp_columns(std::integer_sequence<int, Js...>, f
     g.kc[r] = live ? (uint32_t)(256 - (r * 64 + lane)) : 0u;
   }
   g.sel = -1;
+  g.selslot = -1;
   g.done = false;
   (gepp_step<Js>(v, rid, w, lane, rows, r0, final, g, fs), ...);
+  selslot = g.selslot;
   return g.sel;
 }
 
@@ -169,8 +184,8 @@ __device__ __forceinline__ int gepp_columns(std::integer_sequence<int, Js...>, f
 // like L21.  The pivot reciprocal is v_rcp (the multipliers here only steer
 // the choice of rows).
 __device__ __forceinline__ int wave_gepp(float (&v)[kTsR][kTsW], const int (&rid)[kTsR], int w, int lane,
-                                         float (*rows)[kTsW], int r0, bool final, FinalState& fs) {
-  return gepp_columns(std::make_integer_sequence<int, kTsW>(), v, rid, w, lane, rows, r0, final, fs);
+                                         float (*rows)[kTsW], int r0, bool final, FinalState& fs, int& selslot) {
+  return gepp_columns(std::make_integer_sequence<int, kTsW>(), v, rid, w, lane, rows, r0, final, fs, selslot);
 }
 
 // The wavefront's rows (slot r * 64 + lane holds row rid[r]; srow[g] is the
@@ -229,6 +244,7 @@ __device__ __forceinline__ void load_rows(float (&v)[kTsR][kTsW], const int (&ri
 
 struct TsluOut {
   float u[kTsW][kTsW];     // U11 (row j from column j on, zeros before), in pivot order
+  float asel[kTsW][kTsW];  // the pivot rows' original leaf values, in pivot order (L11 = asel U11^-1)
   int32_t sel[kTsW];       // original row of pivot j
   int32_t move_pos[kTsW];  // rows below the top that receive a displaced top row
   int32_t move_src[kTsW];  // ... which one (0 .. w-1)
@@ -238,11 +254,13 @@ struct TsluOut {
 // sets (or merge groups) of the next level up
 __host__ __device__ __forceinline__ int next_count(int s) { return (s + kTsFan - 1) / kTsFan; }
 
+// The tournament for one leaf (VEC: the final merge takes asel, the pivot
+// rows' original values, from its own LDS staging area).
 template <bool VEC>
 __global__ void __launch_bounds__(64) tslu_tournament_kernel(const float* __restrict__ A, int lda, int n, int r0,
                                                             int c0, int w, int32_t* __restrict__ counters,
-                                                            int32_t* __restrict__ cand, TsluOut* __restrict__ out,
-                                                            int32_t* __restrict__ ipiv, int32_t* __restrict__ info) {
+                                                            int32_t* __restrict__ cand, TsluOut* __restrict__ out, int32_t* __restrict__ ipiv,
+                                                            int32_t* __restrict__ info) {
   __shared__ __attribute__((aligned(16))) float rows[kTsW][kTsW];
   __shared__ __attribute__((aligned(16))) float stg[VEC ? kTsRows : 1][kStgLd];
   const int lane = threadIdx.x;
@@ -270,7 +288,8 @@ __global__ void __launch_bounds__(64) tslu_tournament_kernel(const float* __rest
   for (;;) {
     load_rows<VEC>(v, rid, srow, A, lda, r0, c0, w, lane, stg);
     const bool final = cnt == 1;
-    const int sel = wave_gepp(v, rid, w, lane, rows, r0, final, fs);
+    int selslot;
+    const int sel = wave_gepp(v, rid, w, lane, rows, r0, final, fs, selslot);
     if (final) {
       // U11, the pivots, the interchanges and the moves for the apply kernel
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -282,6 +301,21 @@ __global__ void __launch_bounds__(64) tslu_tournament_kernel(const float* __rest
       if (lane < w) {
         out->sel[lane] = sel;
         ipiv[r0 + lane] = fs.q;
+        if constexpr (VEC) {
+          const float* sv = stg[selslot < 0 ? 0 : selslot];
+#pragma unroll
+          for (int c = 0; c < kTsW; c += 4) {
+            const float4 x = *(const float4*)&sv[c];
+            *(float4*)&out->asel[lane][c] = selslot < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : x;
+          }
+        } else {
+          const float* sp = A + (int64_t)(sel < 0 ? r0 : sel) * lda + c0;
+          float a[kTsW];
+#pragma unroll
+          for (int c = 0; c < kTsW; c++) a[c] = sp[min(c, w - 1)];
+#pragma unroll
+          for (int c = 0; c < kTsW; c++) out->asel[lane][c] = c < w ? a[c] : 0.0f;
+        }
       }
       const unsigned long long mv = __ballot(fs.bp >= 0);
       if (fs.bp >= 0) {
@@ -340,24 +374,34 @@ __device__ __forceinline__ void subst_columns(std::integer_sequence<int, Ks...>,
   (subst_step<Ks>(x, U, dinv), ...);
 }
 
-// Workgroup 0 writes every row whose contents move: the w pivot positions
-// (L11 from the selected rows' original values, U11 from the merge) and the
-// positions below that receive a displaced top row (its L21).  It reads
-// only rows it writes itself, before writing any, so no copy is needed.
-// Every other workgroup takes 256 rows of L21 = A21 U11^-1 in place (rows
-// that receive a displaced top row left to workgroup 0), staged through LDS
-// so loads and stores are coalesced (VEC).  Workgroup 0 also clears the
-// merge counters for the next leaf.
+// Three kinds of workgroups, all independent of each other (each writes
+// what no other reads):
+//  * workgroup 0 writes every row whose contents move, in the leaf's
+//    columns: the w pivot positions (L11 from the selected rows' original
+//    values, U11 from the merge) and the positions below that receive a
+//    displaced top row (its L21).  It reads only rows it writes itself,
+//    before writing any.  It also clears the merge counters for the next leaf;
+//  * workgroups 1 .. l21_blocks take 256 rows each of L21 = A21 U11^-1 in
+//    place (rows that receive a displaced top row left to workgroup 0),
+//    staged through LDS so loads and stores are coalesced (VEC);
+//  * the rest take 32 of the panel's other columns (pc0 .. pc1 outside the
+//    leaf) each: they apply the leaf's row moves there and, right of the
+//    leaf, turn the new top rows into U12 = L11^-1 A12 (forward substitution
+//    with L11 = asel U11^-1).  After this kernel the panel's trailing
+//    columns need only A22 -= L21 U12.
 template <bool VEC>
 __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__ A, int lda, int n, int r0, int c0,
                                                             int w, const TsluOut* __restrict__ out,
-                                                            int32_t* __restrict__ counters, int ncounters) {
+                                                            int32_t* __restrict__ counters, int ncounters,
+                                                            int l21_blocks, int pc0, int pc1) {
   __shared__ __attribute__((aligned(16))) float U[kTsW][kTsW];
   __shared__ float dinv[kTsW];
   __shared__ int mpos[kTsW], msrc[kTsW], psel[kTsW];
   __shared__ int nm;
   __shared__ uint32_t movedbits[kApplyT / 32];  // this workgroup's rows that workgroup 0 writes
   __shared__ __attribute__((aligned(16))) float stg[VEC ? kApplyT : 1][kStgLd];
+  __shared__ float L11[kTsW][kTsW + 1];         // strips: L11, then U12 in place of A12
+  __shared__ float a12[kTsW][kTsW + 1];
   const int t = threadIdx.x;
   if (t < kApplyT / 32) movedbits[t] = 0u;
   for (int e = t; e < kTsW * kTsW; e += kApplyT) U[e / kTsW][e % kTsW] = out->u[e / kTsW][e % kTsW];
@@ -370,9 +414,10 @@ __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__
   }
   if (t == 0) nm = out->nmoves;
   const bool special = blockIdx.x == 0;
+  const bool strip = (int)blockIdx.x > l21_blocks;
   const int i0 = r0 + w + (blockIdx.x - 1) * kApplyT;
   __syncthreads();
-  if (!special && t < nm) {
+  if (!special && !strip && t < nm) {
     const int rel = mpos[t] - i0;
     if (rel >= 0 && rel < kApplyT) atomicOr(&movedbits[rel >> 5], 1u << (rel & 31));
   }
@@ -382,7 +427,11 @@ __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__
   int dst = -1;
   const int seg = t & 7;
   const bool segok = seg * 4 < w;
-  if (special) {
+  if (strip) {
+    // thread t < w: row t of L11 from the selected row's original values
+#pragma unroll
+    for (int c = 0; c < kTsW; c++) x[c] = t < w ? out->asel[t][c] : 0.0f;
+  } else if (special) {
     for (int e = t; e < ncounters; e += kApplyT) counters[e] = 0;
     // thread t < w: pivot position r0 + t; w <= t < w + nm: moved position
     int src = -1;
@@ -433,6 +482,56 @@ __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__
     }
   }
   subst_columns(std::make_integer_sequence<int, kTsW>(), x, U, dinv);
+  if (strip) {
+    // columns cs .. ce of the panel, left of the leaf or right of it
+    const int s = (int)blockIdx.x - 1 - l21_blocks;
+    const int nleft = (c0 - pc0 + kTsW - 1) / kTsW;
+    const bool right = s >= nleft;
+    const int cs = right ? c0 + w + (s - nleft) * kTsW : pc0 + s * kTsW;
+    const int ncols = min(kTsW, (right ? pc1 : c0) - cs);
+    const int moves = w + nm;
+    if (t < w) {
+#pragma unroll
+      for (int j = 0; j < kTsW; j++) L11[t][j] = j < t ? x[j] : 0.0f;
+    }
+    // every moved row's values in these columns (move q < w: pivot position
+    // r0 + q gets row psel[q]; q >= w: mpos[q - w] gets top row r0 + msrc[..])
+    constexpr int kPer = 2 * kTsW * kTsW / kApplyT;
+    float val[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+      const int idx = t + kApplyT * e, q = idx / kTsW, c = idx % kTsW;
+      const int src = q < w ? psel[q] : r0 + msrc[min(q - w, kTsW - 1)];
+      val[e] = (q < moves && c < ncols) ? A[(int64_t)src * lda + cs + c] : 0.0f;
+    }
+    if (right) {
+#pragma unroll
+      for (int e = 0; e < kPer; e++) {
+        const int idx = t + kApplyT * e, q = idx / kTsW, c = idx % kTsW;
+        if (q < w) a12[q][c] = val[e];
+      }
+    }
+    __syncthreads();  // every row this workgroup moves is read (and L11 is set) before any is written
+    if (right) {
+      if (t < ncols) {  // U12 = L11^-1 A12, column t, in pivot order
+        for (int r = 1; r < w; r++) {
+          float acc = a12[r][t];
+          for (int j = 0; j < r; j++) acc = fmaf(-L11[r][j], a12[j][t], acc);
+          a12[r][t] = acc;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+      const int idx = t + kApplyT * e, q = idx / kTsW, c = idx % kTsW;
+      if (q < moves && c < ncols) {
+        const int d = q < w ? r0 + q : mpos[min(q - w, kTsW - 1)];
+        A[(int64_t)d * lda + cs + c] = (right && q < w) ? a12[q][c] : val[e];
+      }
+    }
+    return;
+  }
   if (special || !VEC) {
     const bool moved = !special && ((movedbits[t >> 5] >> (t & 31)) & 1u);  // a moved row is workgroup 0's
     if (dst < 0 || dst >= n || moved) return;
@@ -457,51 +556,130 @@ __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__
   }
 }
 
+// The panel's interchanges over the columns c_lo .. c_hi (outside the
+// panel), once per panel: leaf by leaf in order, each thread moves one column -- all of a leaf's
+// moved rows loaded (w pivot positions from the selected rows, nmoves
+// displaced top rows into the vacated positions), then stored, so the
+// result is the leaf's interchange sequence applied in order.  The moves are
+// the tournament's own records, read once per workgroup into LDS.
+constexpr int kTsluMaxLeaves = 32;  // leaves per panel (panel width <= 1024)
+__global__ void __launch_bounds__(64) tslu_swap_outside_kernel(float* __restrict__ A, int lda, int k, int nb,
+                                                               int c_lo, int c_hi, const TsluOut* __restrict__ outs,
+                                                               int leaves) {
+  __shared__ int msrc_s[kTsluMaxLeaves][2 * kTsW], mdst_s[kTsluMaxLeaves][2 * kTsW], mcnt_s[kTsluMaxLeaves];
+  const int t = threadIdx.x;
+  for (int e = t; e < leaves * 2 * kTsW; e += blockDim.x) {
+    const int li = e / (2 * kTsW), q = e % (2 * kTsW);
+    const TsluOut& o = outs[li];
+    const int r0 = k + li * kTsW, w = min(kTsW, k + nb - r0), nm = o.nmoves;
+    int src = -1, dst = -1;
+    if (q < w) {
+      src = o.sel[q];
+      dst = r0 + q;
+    } else if (q < w + nm) {
+      src = r0 + o.move_src[q - w];
+      dst = o.move_pos[q - w];
+    }
+    msrc_s[li][q] = src < 0 ? r0 : src;  // unused entries read a valid row and store nothing
+    mdst_s[li][q] = dst;
+    if (q == 0) mcnt_s[li] = w + nm;
+  }
+  __syncthreads();
+  const int c = c_lo + (int)(blockIdx.x * blockDim.x) + t;
+  if (c >= c_hi) return;
+  for (int li = 0; li < leaves; li++) {
+    float v[2 * kTsW];
+#pragma unroll
+    for (int q = 0; q < 2 * kTsW; q++) v[q] = A[(int64_t)msrc_s[li][q] * lda + c];
+    const int cnt = mcnt_s[li];
+#pragma unroll
+    for (int q = 0; q < 2 * kTsW; q++)
+      if (q < cnt) A[(int64_t)mdst_s[li][q] * lda + c] = v[q];
+  }
+}
+
 }  // namespace
 
 // Scratch of the tournament: merge counters (zero between leaves), the
 // final merge's output, the candidate sets of every level.
 constexpr int kTsluMaxCounters = 4096;
+constexpr size_t kOutsBytes = ((sizeof(TsluOut) * kTsluMaxLeaves + 255) & ~(size_t)255);
 size_t tslu_scratch_bytes(int n) {
   const int G = ceil_div(n, kTsRows);
   size_t sets = 0;
   for (int s = G; s >= 1; s = s == 1 ? 0 : next_count(s)) sets += (size_t)s;
-  return 4 * (size_t)kTsluMaxCounters + ((sizeof(TsluOut) + 255) & ~(size_t)255) + 4 * sets * kTsW + 256;
+  return 4 * (size_t)kTsluMaxCounters + kOutsBytes + 4 * sets * kTsW + 256;
 }
 
-int tslu_leaf_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int r0, int c0,
-                     int w, int32_t* ipiv, int32_t* info) {
-  const int m = n - r0;
-  if (m <= 0 || w <= 0) return NMX_OK;
-  if (w > kTsW || m < w) return NMX_ERR_UNSUPPORTED;
+// One panel (columns k .. k+nb, rows k .. n) as a flat sequence of 32-column
+// leaves: tournament, apply (which also moves the panel's other columns and
+// forms U12), and one GEMM for the panel's columns right of the leaf; then
+// (swap_outside) the panel's interchanges over every column outside it.
+int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi) {
+  if (c_hi <= c_lo || nb <= 0) return NMX_OK;
+  if (c_lo < k + nb && c_hi > k) return NMX_ERR_UNSUPPORTED;  // the panel's own columns are moved by the apply
+  const TsluOut* outs = (const TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters);
+  ProfScope ps(ctx, K_LASWP, 0.0, 16.0 * nb * (double)(c_hi - c_lo));
+  tslu_swap_outside_kernel<<<ceil_div(c_hi - c_lo, 64), 64, 0, ctx->stream>>>(A, lda, k, nb, c_lo, c_hi, outs,
+                                                                                ceil_div(nb, kTsW));
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+int tslu_panel_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int k, int nb,
+                      int32_t* ipiv, int32_t* info, int swap_outside) {
+  if (nb <= 0 || n - k <= 0) return NMX_OK;
   if (tslu_scratch_bytes(n) > scratch_bytes) return NMX_ERR_UNSUPPORTED;
-  const int G = ceil_div(m, kTsRows);
-  int ncounters = 0;
-  for (int s = G; s > 1; s = next_count(s)) ncounters += next_count(s);
-  if (ncounters > kTsluMaxCounters) return NMX_ERR_UNSUPPORTED;
+  if (ceil_div(nb, kTsW) > kTsluMaxLeaves) return NMX_ERR_UNSUPPORTED;
   int32_t* counters = (int32_t*)scratch;
-  TsluOut* out = (TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters);
-  int32_t* cand = (int32_t*)((char*)out + ((sizeof(TsluOut) + 255) & ~(size_t)255));
-  // 16-byte row pieces: every row start and the leaf's first column aligned
-  const bool vec = lda % 4 == 0 && c0 % 4 == 0 && w % 4 == 0 && ((uintptr_t)A & 15) == 0;
-  {
-    ProfScope ps(ctx, K_LEAF, (double)m * w * w, 4.0 * (double)m * w);
-    if (vec)
-      tslu_tournament_kernel<true><<<G, 64, 0, ctx->stream>>>(A, lda, n, r0, c0, w, counters, cand, out, ipiv, info);
-    else
-      tslu_tournament_kernel<false><<<G, 64, 0, ctx->stream>>>(A, lda, n, r0, c0, w, counters, cand, out, ipiv, info);
-    NMX_HIP(ctx, hipGetLastError());
+  TsluOut* outs = (TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters);
+  int32_t* cand = (int32_t*)((char*)outs + kOutsBytes);
+  const int pc0 = k, pc1 = k + nb;
+  for (int c0 = k; c0 < pc1; c0 += kTsW) {
+    TsluOut* out = outs + (c0 - k) / kTsW;
+    const int r0 = c0, w = std::min(kTsW, pc1 - c0), m = n - r0;
+    if (m < w) return NMX_ERR_UNSUPPORTED;
+    const int G = ceil_div(m, kTsRows);
+    int ncounters = 0;
+    for (int s = G; s > 1; s = next_count(s)) ncounters += next_count(s);
+    if (ncounters > kTsluMaxCounters) return NMX_ERR_UNSUPPORTED;
+    // 16-byte row pieces: every row start and the leaf's first column aligned
+    const bool vec = lda % 4 == 0 && c0 % 4 == 0 && w % 4 == 0 && ((uintptr_t)A & 15) == 0;
+    {
+      ProfScope ps(ctx, K_LEAF, (double)m * w * w, 4.0 * (double)m * w);
+      if (vec)
+        tslu_tournament_kernel<true><<<G, 64, 0, ctx->stream>>>(A, lda, n, r0, c0, w, counters, cand, out, ipiv,
+                                                                info);
+      else
+        tslu_tournament_kernel<false><<<G, 64, 0, ctx->stream>>>(A, lda, n, r0, c0, w, counters, cand, out, ipiv,
+                                                                 info);
+      NMX_HIP(ctx, hipGetLastError());
+    }
+    {
+      const int other = (c0 - pc0) + (pc1 - c0 - w);
+      ProfScope ps(ctx, K_MP_PANEL, (double)m * w * w, 8.0 * (double)m * w + 16.0 * w * (double)other);
+      const int l21_blocks = ceil_div(m - w, kApplyT);
+      const int strips = ceil_div(c0 - pc0, kTsW) + ceil_div(pc1 - c0 - w, kTsW);
+      const int blocks = 1 + l21_blocks + strips;
+      if (vec)
+        tslu_apply_kernel<true><<<blocks, kApplyT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, out, counters, ncounters,
+                                                                     l21_blocks, pc0, pc1);
+      else
+        tslu_apply_kernel<false><<<blocks, kApplyT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, out, counters, ncounters,
+                                                                      l21_blocks, pc0, pc1);
+      NMX_HIP(ctx, hipGetLastError());
+    }
+    const int rows = m - w, cols = pc1 - c0 - w;
+    if (rows > 0 && cols > 0)
+      NMX_TRY(gemm_launch<float>(ctx, 0, 0, rows, cols, w, -1.0f, A + (int64_t)(r0 + w) * lda + c0, lda,
+                                 A + (int64_t)r0 * lda + c0 + w, lda, 1.0f, A + (int64_t)(r0 + w) * lda + c0 + w, lda,
+                                 0));
   }
-  {
-    ProfScope ps(ctx, K_MP_PANEL, (double)m * w * w, 8.0 * (double)m * w);
-    const int blocks = 1 + ceil_div(m - w, kApplyT);
-    if (vec)
-      tslu_apply_kernel<true><<<blocks, kApplyT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, out, counters, ncounters);
-    else
-      tslu_apply_kernel<false><<<blocks, kApplyT, 0, ctx->stream>>>(A, lda, n, r0, c0, w, out, counters, ncounters);
-    NMX_HIP(ctx, hipGetLastError());
+  if (swap_outside) {
+    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, 0, k));
+    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, k + nb, n));
   }
-  return laswp_launch<float>(ctx, A, lda, 0, n, c0, c0 + w, ipiv, r0, r0 + w);
+  return NMX_OK;
 }
 
 // Clears the merge counters once, when the scratch is allocated.
