@@ -41,9 +41,15 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
 # n = 16384 for the LU paths, the 20,000-system batch for the batched leg.
 # Keyed by (kernel class, n or batch count); another size reports null.
 MEASURED_TRAFFIC = {
-    ("getf2_leaf", 16384): 13.59 * 2 ** 20,  # mixed-precision path, FP32 leaf (both row layouts)
-    ("mp_gemm", 16384): 823.45 * 2 ** 20,
-    ("batched_solve", 20000): 8 * 4462.17 * 2 ** 20,  # the eight class launches of one call
+    # mixed-precision path: the tournament (leaf class), averaged over the factorization's leaves
+    ("getf2_leaf", 16384): (2.16 + 0.01) * 2 ** 20,
+    ("mp_gemm", 16384): (249.53 + 170.69) * 2 ** 20,  # next-panel and look-ahead parts, averaged
+    # plain FP64 path: the 128-tile trailing updates and the 64-tile in-panel updates, per class launch
+    ("gemm", 16384): 108.68 * 2 ** 20,
+    ("batched_solve", 20000): 35686.44 * 2 ** 20,  # the eight class launches of one call
+    ("c2_lu_gemm", 8192): 28.04 * 2 ** 20,  # per class launch, FP64 LU + 16-RHS solve
+    ("batched_eigh", 65536): (405.22 + 408.13 + 103.09 + 104.12) * 2 ** 20,  # both class launches
+    ("batched_lstsq", 4096): (669.27 + 0.50) * 2 ** 20,
 }
 
 
@@ -201,7 +207,7 @@ def timed_steps(step, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def class_roofline(prof, name, bound, flops=None, nbytes=None, peak_key=None) -> dict:
+def class_roofline(prof, name, bound, flops=None, nbytes=None, peak_key=None, traffic=None) -> dict:
     """Roofline object of one kernel class from the library's per-launch HIP
     events: `flops` / `nbytes` are the algorithmic work of the whole step
     (None: the profiler's own per-launch counts)."""
@@ -218,7 +224,7 @@ def class_roofline(prof, name, bound, flops=None, nbytes=None, peak_key=None) ->
         peak = PEAK["hbm_GBs"]
         unit = "GB/s"
     return {"kernel": name, "bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
-            "traffic": None, "launches_per_step": rec["launches"], "kernel_ms": rec["ms"],
+            "traffic": traffic, "launches_per_step": rec["launches"], "kernel_ms": rec["ms"],
             "avg_launch_ms": rec["ms"] / max(rec["launches"], 1)}
 
 
@@ -266,7 +272,8 @@ def c2_leg(args, torch) -> dict:
         dt = timed_steps(step, args.steps, 1, torch)
         eta = backward_error(a, x, bm)
         prof = profile_step(ctx, step)
-        roof = class_roofline(prof, "gemm", "mfma", peak_key="fp64_TFs")
+        roof = class_roofline(prof, "gemm", "mfma", peak_key="fp64_TFs",
+                              traffic=MEASURED_TRAFFIC.get(("c2_lu_gemm", n)) if name == "lu" else None)
         roof["share_of_step"] = prof["gemm"]["ms"] / (dt * 1e3)
         cpu = None
         if not args.no_cpu:
@@ -345,7 +352,8 @@ def c5_legs(args, torch) -> dict:
     prof = profile_step(ctx, eigh_step)
     nn = n.astype(np.float64)
     flops = float((12 * nn * (nn - 1) / 2 * (18 * nn + 15)).sum())  # numax's rotation count x flops
-    roof = class_roofline(prof, "batched_eigh", "valu", flops=flops, peak_key="fp32_TFs")
+    roof = class_roofline(prof, "batched_eigh", "valu", flops=flops, peak_key="fp32_TFs",
+                          traffic=MEASURED_TRAFFIC.get(("batched_eigh", len(n))))
     cpu = None
     if not args.no_cpu:
         oracle.set_threads(threads)
@@ -381,7 +389,7 @@ def c5_legs(args, torch) -> dict:
     dt = timed_steps(lstsq_step, args.steps, 1, torch)
     prof = profile_step(ctx, lstsq_step)
     nbytes = 4.0 * (m * k + m + k) * cnt
-    roof = class_roofline(prof, "batched_lstsq", "hbm", nbytes=nbytes)
+    roof = class_roofline(prof, "batched_lstsq", "hbm", nbytes=nbytes, traffic=MEASURED_TRAFFIC.get(("batched_lstsq", cnt)))
     cpu = None
     if not args.no_cpu:
         s = args.lstsq_cpu_sample

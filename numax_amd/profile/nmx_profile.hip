@@ -3,10 +3,14 @@
 // profiler sees is either the library's or the calibration copy below.
 //
 //   nmx_profile copy  BYTES      plain float4 copy of BYTES (HBM calibration)
-//   nmx_profile lu    N          FP64 getrf + getrs, A ~ N(0,1), one RHS
+//   nmx_profile lu    N [NRHS]   FP64 getrf + getrs, A ~ N(0,1), NRHS (1) right-hand sides
+//   nmx_profile chol  N [NRHS]   FP64 potrf + potrs, A = G G^T / n + I, NRHS (1) right-hand sides
 //   nmx_profile mpir  N [PREC]   mixed-precision LU-IR, A ~ N(0,1) + 0 I
 //   nmx_profile batch COUNT      uneven batch, n ~ U{8..256}, half LU / half
 //                                Cholesky, FP32 factor + FP64 refinement
+//   nmx_profile eigh  COUNT      batched Jacobi eigh, n in {24, 32, 48, 64} in equal
+//                                shares, (G + G^T) / 2, FP32, 12 sweeps, vectors
+//   nmx_profile lstsq COUNT      batched least squares, A 1024 x 32 ~ N(0,1), FP32
 //
 // Each mode runs its path twice (a warm-up, then the measured call) and
 // prints one line per step.
@@ -59,22 +63,119 @@ static int run_copy(long long bytes) {
   return 0;
 }
 
-static int run_lu(nmx_ctx* ctx, int n) {
+static int run_lu(nmx_ctx* ctx, int n, int nrhs) {
   double *A, *LU, *b;
   int32_t *ipiv, *info;
   HCHECK(hipMalloc(&A, sizeof(double) * n * n));
   HCHECK(hipMalloc(&LU, sizeof(double) * n * n));
-  HCHECK(hipMalloc(&b, sizeof(double) * n));
+  HCHECK(hipMalloc(&b, sizeof(double) * n * nrhs));
   HCHECK(hipMalloc(&ipiv, sizeof(int32_t) * n));
   HCHECK(hipMalloc(&info, sizeof(int32_t)));
   CHECK(nmx_fill_random_f64(ctx, 1, 7, 0, (int64_t)n * n, 0, 0, A));
   for (int it = 0; it < 2; it++) {
-    CHECK(nmx_fill_random_f64(ctx, 1, 8, 0, n, 0, 0, b));
+    CHECK(nmx_fill_random_f64(ctx, 1, 8, 0, (int64_t)n * nrhs, 0, 0, b));
     HCHECK(hipMemcpy(LU, A, sizeof(double) * n * n, hipMemcpyDeviceToDevice));
     CHECK(nmx_getrf_f64(ctx, n, LU, n, ipiv, info, 256, 0));
-    CHECK(nmx_getrs_f64(ctx, n, 1, LU, n, ipiv, b, 1));
+    CHECK(nmx_getrs_f64(ctx, n, nrhs, LU, n, ipiv, b, nrhs));
     CHECK(nmx_sync(ctx));
-    printf("lu step %d: n=%d done\n", it, n);
+    printf("lu step %d: n=%d nrhs=%d done\n", it, n, nrhs);
+    fflush(stdout);
+  }
+  return 0;
+}
+
+__global__ void add_identity_kernel(int n, double* A, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(int64_t)i * n + i] += scale;
+}
+
+static int run_chol(nmx_ctx* ctx, int n, int nrhs) {
+  double *G, *A, *L, *b;
+  int32_t* info;
+  HCHECK(hipMalloc(&G, sizeof(double) * n * n));
+  HCHECK(hipMalloc(&A, sizeof(double) * n * n));
+  HCHECK(hipMalloc(&L, sizeof(double) * n * n));
+  HCHECK(hipMalloc(&b, sizeof(double) * n * nrhs));
+  HCHECK(hipMalloc(&info, sizeof(int32_t)));
+  CHECK(nmx_fill_random_f64(ctx, 1, 9, 0, (int64_t)n * n, 0, 0, G));
+  CHECK(nmx_gemm_f64(ctx, 0, 1, n, n, n, 1.0 / n, G, n, G, n, 0.0, A, n));
+  add_identity_kernel<<<(n + 255) / 256, 256>>>(n, A, 1.0);
+  HCHECK(hipGetLastError());
+  HCHECK(hipDeviceSynchronize());
+  for (int it = 0; it < 2; it++) {
+    CHECK(nmx_fill_random_f64(ctx, 1, 8, 0, (int64_t)n * nrhs, 0, 0, b));
+    HCHECK(hipMemcpy(L, A, sizeof(double) * n * n, hipMemcpyDeviceToDevice));
+    CHECK(nmx_potrf_f64(ctx, n, L, n, info, 256));
+    CHECK(nmx_potrs_f64(ctx, n, nrhs, L, n, b, nrhs));
+    CHECK(nmx_sync(ctx));
+    int h = 0;
+    HCHECK(hipMemcpy(&h, info, 4, hipMemcpyDeviceToHost));
+    printf("chol step %d: n=%d nrhs=%d info %d\n", it, n, nrhs, h);
+    fflush(stdout);
+  }
+  return 0;
+}
+
+static int run_eigh(nmx_ctx* ctx, int count) {
+  std::mt19937_64 rng(13);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  const int sizes[4] = {24, 32, 48, 64};
+  std::vector<int32_t> n(count);
+  std::vector<int64_t> oa(count), ow(count);
+  int64_t ae = 0, we = 0;
+  for (int p = 0; p < count; p++) {
+    n[p] = sizes[p % 4];
+    oa[p] = ae;
+    ow[p] = we;
+    ae += (int64_t)n[p] * n[p];
+    we += n[p];
+  }
+  std::vector<float> a(ae);
+  for (int p = 0; p < count; p++) {
+    const int m = n[p];
+    float* ap = a.data() + oa[p];
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j <= i; j++) ap[i * m + j] = ap[j * m + i] = nd(rng);
+  }
+  int32_t* dn;
+  int64_t *doa, *dow;
+  float *dA, *dW, *dV;
+  HCHECK(hipMalloc(&dn, 4 * count));
+  HCHECK(hipMalloc(&doa, 8 * count));
+  HCHECK(hipMalloc(&dow, 8 * count));
+  HCHECK(hipMalloc(&dA, 4 * ae));
+  HCHECK(hipMalloc(&dW, 4 * we));
+  HCHECK(hipMalloc(&dV, 4 * ae));
+  HCHECK(hipMemcpy(dn, n.data(), 4 * count, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(doa, oa.data(), 8 * count, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(dow, ow.data(), 8 * count, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(dA, a.data(), 4 * ae, hipMemcpyHostToDevice));
+  for (int it = 0; it < 2; it++) {
+    CHECK(nmx_batched_syevj_f32(ctx, count, dn, doa, dow, dA, ae, dW, we, dV, 12));
+    CHECK(nmx_sync(ctx));
+    printf("eigh step %d: %d matrices\n", it, count);
+    fflush(stdout);
+  }
+  return 0;
+}
+
+static int run_lstsq(nmx_ctx* ctx, int count) {
+  const int m = 1024, n = 32;
+  float *A, *B, *X;
+  HCHECK(hipMalloc(&A, sizeof(float) * (size_t)count * m * n));
+  HCHECK(hipMalloc(&B, sizeof(float) * (size_t)count * m));
+  HCHECK(hipMalloc(&X, sizeof(float) * (size_t)count * n));
+  std::mt19937_64 rng(17);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  std::vector<float> a((size_t)count * m * n), b((size_t)count * m);
+  for (auto& v : a) v = nd(rng);
+  for (auto& v : b) v = nd(rng);
+  HCHECK(hipMemcpy(A, a.data(), 4 * a.size(), hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(B, b.data(), 4 * b.size(), hipMemcpyHostToDevice));
+  for (int it = 0; it < 2; it++) {
+    CHECK(nmx_batched_gels_f32(ctx, count, m, n, A, B, X));
+    CHECK(nmx_sync(ctx));
+    printf("lstsq step %d: %d problems %d x %d\n", it, count, m, n);
     fflush(stdout);
   }
   return 0;
@@ -172,7 +273,8 @@ static int run_batch(nmx_ctx* ctx, int count) {
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: nmx_profile copy BYTES | lu N | mpir N [PREC] | batch COUNT\n");
+    fprintf(stderr, "usage: nmx_profile copy BYTES | lu N [NRHS] | chol N [NRHS] | mpir N [PREC] | batch COUNT | "
+                    "eigh COUNT | lstsq COUNT\n");
     return 2;
   }
   const char* mode = argv[1];
@@ -180,7 +282,10 @@ int main(int argc, char** argv) {
   nmx_ctx* ctx;
   CHECK(nmx_ctx_create(0, nullptr, &ctx));
   int rc = 2;
-  if (!strcmp(mode, "lu")) rc = run_lu(ctx, atoi(argv[2]));
+  if (!strcmp(mode, "lu")) rc = run_lu(ctx, atoi(argv[2]), argc > 3 ? atoi(argv[3]) : 1);
+  else if (!strcmp(mode, "chol")) rc = run_chol(ctx, atoi(argv[2]), argc > 3 ? atoi(argv[3]) : 1);
+  else if (!strcmp(mode, "eigh")) rc = run_eigh(ctx, atoi(argv[2]));
+  else if (!strcmp(mode, "lstsq")) rc = run_lstsq(ctx, atoi(argv[2]));
   else if (!strcmp(mode, "mpir")) rc = run_mpir(ctx, atoi(argv[2]), argc > 3 ? atoi(argv[3]) : 2);
   else if (!strcmp(mode, "batch")) rc = run_batch(ctx, atoi(argv[2]));
   else fprintf(stderr, "unknown mode %s\n", mode);
