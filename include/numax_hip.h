@@ -32,6 +32,11 @@ extern "C" {
 #define NMX_ERR_UNSUPPORTED 3 /* size or option outside what the kernels handle */
 #define NMX_ERR_SYNC_TIMEOUT 4 /* a cross-workgroup wait hit its spin bound */
 #define NMX_ERR_NOT_CONVERGED 5 /* refinement did not reach its target */
+#define NMX_ERR_IO 6          /* a file could not be opened, read or written */
+#define NMX_ERR_FORMAT 7      /* not a .npy / NMX1 file, an unsupported .npy version, or a truncated header */
+#define NMX_ERR_DTYPE 8       /* the file's dtype is not the one requested, or not one numax stores */
+#define NMX_ERR_SHAPE 9       /* the file's rank, shape or payload size disagrees with the request */
+#define NMX_ERR_LAYOUT 10     /* a Fortran-ordered or big-endian .npy file (only row-major little-endian is read) */
 
 typedef struct nmx_ctx nmx_ctx;
 
@@ -348,6 +353,33 @@ int nmx_qr_r_f32(nmx_ctx* ctx, int n, const float* QR, int lda, float* R, int ld
  * place -- X is B's first n rows. */
 int nmx_gels_f64(nmx_ctx* ctx, int m, int n, int nrhs, double* A, int lda, double* tau, double* B, int ldb, int block);
 int nmx_gels_f32(nmx_ctx* ctx, int m, int n, int nrhs, float* A, int lda, float* tau, float* B, int ldb, int block);
+
+/* ---- tensor files: .npy and NMX1 ---------------------------------------------
+ * numax.io.numpy.save / load / load_dyn (numax/io/npy.mojo:262-453) and
+ * numax.io.nmx.save / load (numax/io/io.mojo:59-194): `format` 0 = .npy
+ * (read: format 1.x / 2.x / 3.x headers; written: 1.0, padded as
+ * numpy.save pads, so the file is byte-identical to numpy.save's), 1 = NMX1
+ * ("NMX1", a length-prefixed dtype name, a rank byte, little-endian int64
+ * dims, the row-major payload).  dtype codes: 0 float16, 1 float32,
+ * 2 float64, 3 int8, 4 int16, 5 int32, 6 int64, 7 uint8, 8 uint16,
+ * 9 uint32, 10 uint64, 11 bool; rank <= 8.  The payload streams between
+ * the file and device memory through two pinned staging buffers (the file
+ * read into one while the other is copied on the context's stream); load
+ * and save return when the data has landed (they synchronize the stream).
+ * Errors: NMX_ERR_IO, _FORMAT (bad magic, e.g. an .npz archive; unsupported
+ * version; truncated header), _DTYPE, _SHAPE (rank, dims, or a payload
+ * longer or shorter than the header says), _LAYOUT (Fortran order,
+ * big-endian). */
+/* The file's dtype, rank and dims (dims: room for 8), no device work:
+ * the shape load_dyn takes from the file. */
+int nmx_tensor_file_info(const char* path, int format, int* dtype, int* rank, int64_t* dims);
+/* Reads the file into device memory dst, which holds prod(dims) elements;
+ * dtype, rank and dims must be the file's exactly (numax's typed load). */
+int nmx_tensor_file_load(nmx_ctx* ctx, const char* path, int format, int dtype, int rank, const int64_t* dims,
+                         void* dst);
+/* Writes device memory src (prod(dims) elements, row-major) to the file. */
+int nmx_tensor_file_save(nmx_ctx* ctx, const char* path, int format, int dtype, int rank, const int64_t* dims,
+                         const void* src);
 
 /* ---- graph capture ----------------------------------------------------------
  * Records the library calls made on the context's stream between

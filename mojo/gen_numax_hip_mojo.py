@@ -16,7 +16,7 @@ HEADER = os.path.join(HERE, "..", "include", "numax_hip.h")
 OUT = os.path.join(HERE, "numax_hip.mojo")
 
 SCALARS = {"int": "Int32", "int64_t": "Int64", "uint64_t": "UInt64", "double": "Float64", "float": "Float32"}
-POINTEES = {"double": "Float64", "float": "Float32", "int32_t": "Int32", "int64_t": "Int64", "uint8_t": "UInt8",
+POINTEES = {"char": "Int8", "int": "Int32", "double": "Float64", "float": "Float32", "int32_t": "Int32", "int64_t": "Int64", "uint8_t": "UInt8",
             "nmx_ir_cfg": "NmxIrCfg", "nmx_ir_report": "NmxIrReport", "nmx_batch_cfg": "NmxBatchCfg"}
 OPAQUE = {"void", "nmx_ctx", "nmx_graph"}
 STRUCTS = {"nmx_ir_cfg": "NmxIrCfg", "nmx_ir_report": "NmxIrReport", "nmx_batch_cfg": "NmxBatchCfg"}

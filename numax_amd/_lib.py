@@ -90,6 +90,9 @@ SIGNATURES = {
     "nmx_batched_slogdet_cholesky_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
     "nmx_map_blocks_solve_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
     "nmx_map_blocks_solve_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
+    "nmx_tensor_file_info": (_c_int, [ctypes.c_char_p, _c_int, _c_vp, _c_vp, _c_vp]),
+    "nmx_tensor_file_load": (_c_int, [_c_vp, ctypes.c_char_p, _c_int, _c_int, _c_int, _c_vp, _c_vp]),
+    "nmx_tensor_file_save": (_c_int, [_c_vp, ctypes.c_char_p, _c_int, _c_int, _c_int, _c_vp, _c_vp]),
     "nmx_batched_gesv": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp,
                                   _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_batched_gesv_noalloc": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,
@@ -146,6 +149,20 @@ def load() -> ctypes.CDLL:
                 fn.argtypes = args
             _lib = lib
     return _lib
+
+
+# status codes (include/numax_hip.h)
+NMX_OK = 0
+NMX_ERR_HIP = 1
+NMX_ERR_ALLOC = 2
+NMX_ERR_UNSUPPORTED = 3
+NMX_ERR_SYNC_TIMEOUT = 4
+NMX_ERR_NOT_CONVERGED = 5
+NMX_ERR_IO = 6
+NMX_ERR_FORMAT = 7
+NMX_ERR_DTYPE = 8
+NMX_ERR_SHAPE = 9
+NMX_ERR_LAYOUT = 10
 
 
 def status_string(status: int) -> str:

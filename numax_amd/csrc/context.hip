@@ -14,6 +14,11 @@ NMX_API const char* nmx_status_string(int status) {
     case NMX_ERR_UNSUPPORTED: return "unsupported size or option";
     case NMX_ERR_SYNC_TIMEOUT: return "cross-workgroup wait timed out";
     case NMX_ERR_NOT_CONVERGED: return "iterative refinement did not converge";
+    case NMX_ERR_IO: return "file could not be opened, read or written";
+    case NMX_ERR_FORMAT: return "not a .npy / NMX1 file, or a truncated or unsupported header";
+    case NMX_ERR_DTYPE: return "the file holds another dtype";
+    case NMX_ERR_SHAPE: return "the file's rank, shape or payload size disagrees with the request";
+    case NMX_ERR_LAYOUT: return "Fortran-ordered or big-endian .npy";
     default: return "unknown status";
   }
 }
