@@ -119,6 +119,22 @@ int nmx_trsm_f64(nmx_ctx* ctx, int upper, int unit, int trans, int n, int nrhs, 
 int nmx_trsm_f32(nmx_ctx* ctx, int upper, int unit, int trans, int n, int nrhs, const float* T, int ldt, float* B,
                  int ldb);
 
+/* ---- symmetric indefinite LDL^T ---------------------------------------------
+ * ldl (numax/linalg/ldl.mojo:59-148): Bunch-Kaufman sytf2 on the lower
+ * triangle (numax/linalg/panel.mojo:2737-2894) in place, one workgroup;
+ * ipiv 1-based as LAPACK's (kp + 1 for a 1 x 1 block swapped with row kp,
+ * -(kp + 1) on both rows of a 2 x 2 block); info = first k + 1 whose column
+ * is exactly zero (the factorization goes on and yields non-finite entries,
+ * as numax's does).  The strict upper triangle is not read. */
+int nmx_sytrf_f64(nmx_ctx* ctx, int n, double* A, int lda, int32_t* ipiv, int32_t* info);
+int nmx_sytrf_f32(nmx_ctx* ctx, int n, float* A, int lda, int32_t* ipiv, int32_t* info);
+/* SciPy's (lu, d, perm) from the packed factor (panel.mojo:2897-3003):
+ * A = lu d lu^T, lu[perm] unit lower triangular, d block diagonal. */
+int nmx_ldl_unpack_f64(nmx_ctx* ctx, int n, const double* LDU, int lda, const int32_t* ipiv, double* L, int ldl,
+                       double* D, int ldd, int64_t* perm);
+int nmx_ldl_unpack_f32(nmx_ctx* ctx, int n, const float* LDU, int lda, const int32_t* ipiv, float* L, int ldl,
+                       float* D, int ldd, int64_t* perm);
+
 /* ---- Cholesky ------------------------------------------------------------
  * cholesky[gpu=True] (numax/linalg/cholesky.mojo:59-396): lower L with
  * L L^T = A reading only A's lower triangle; a non-positive pivot is floored

@@ -90,6 +90,10 @@ SIGNATURES = {
     "nmx_batched_slogdet_cholesky_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
     "nmx_map_blocks_solve_f64": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
     "nmx_map_blocks_solve_f32": (_c_int, [_c_vp, _c_int, _c_int, _c_i64, _c_vp, _c_vp]),
+    "nmx_sytrf_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp]),
+    "nmx_sytrf_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp]),
+    "nmx_ldl_unpack_f64": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_vp, _c_int, _c_vp]),
+    "nmx_ldl_unpack_f32": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_int, _c_vp, _c_int, _c_vp]),
     "nmx_tensor_file_info": (_c_int, [ctypes.c_char_p, _c_int, _c_vp, _c_vp, _c_vp]),
     "nmx_tensor_file_load": (_c_int, [_c_vp, ctypes.c_char_p, _c_int, _c_int, _c_int, _c_vp, _c_vp]),
     "nmx_tensor_file_save": (_c_int, [_c_vp, ctypes.c_char_p, _c_int, _c_int, _c_int, _c_vp, _c_vp]),

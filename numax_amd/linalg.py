@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["LU", "PLU", "NumaxError", "lu_factor", "lu", "inverse", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
+__all__ = ["LU", "PLU", "LDL", "NumaxError", "lu_factor", "lu", "ldl", "inverse", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
            "gemm", "PRECISIONS"]
 
 # trailing-update precisions of the mixed-precision solver (nmx_ir_cfg.precision)
@@ -218,6 +218,44 @@ def inverse(a, block: int = 16) -> torch.Tensor:
     _lib.check(_fn("getri", f.factored.dtype)(_ctx(f.factored).handle, n, _ptr(f.factored), ld, _ptr(f.pivots),
                                                _ptr(out), ld), "inverse")
     return out
+
+
+class LDL:
+    """What numax's ``ldl`` returns (numax/linalg/ldl.mojo:30-56): SciPy's
+    ``(lu, d, perm)`` with ``a = lu @ d @ lu^T`` and ``lu[perm]`` unit lower
+    triangular; ``factored`` / ``ipiv`` are sytf2's packed factor and pivots."""
+
+    def __init__(self, lu, d, perm, factored, ipiv, info):
+        self.lu = lu
+        self.d = d
+        self.perm = perm
+        self.factored = factored
+        self.ipiv = ipiv
+        self.info = info
+
+    def __iter__(self):
+        return iter((self.lu, self.d, self.perm))
+
+
+def ldl(a) -> LDL:
+    """numax ``ldl`` (numax/linalg/ldl.mojo:59-148), ``scipy.linalg.ldl(a)``
+    with ``lower=True``: Bunch-Kaufman ``L D L^T`` of the symmetric ``a``
+    (lower triangle read).  A singular ``a`` does not raise; its zero pivot
+    yields non-finite entries."""
+    work = _device_tensor(a).clone()
+    n = _square(work, "ldl")
+    dev, dt = work.device, work.dtype
+    ld = max(n, 1)
+    ipiv = torch.zeros(ld, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    ctx = _ctx(work)
+    _lib.check(_fn("sytrf", dt)(ctx.handle, n, _ptr(work), ld, _ptr(ipiv), _ptr(info)), "ldl")
+    lu = torch.empty((n, n), dtype=dt, device=dev)
+    d = torch.empty_like(lu)
+    perm = torch.empty(ld, dtype=torch.int64, device=dev)
+    _lib.check(_fn("ldl_unpack", dt)(ctx.handle, n, _ptr(work), ld, _ptr(ipiv), _ptr(lu), ld, _ptr(d), ld, _ptr(perm)),
+               "ldl")
+    return LDL(lu, d, perm[:n], work, ipiv[:n], int(info.item()))
 
 
 def laswp(b, pivots) -> torch.Tensor:

@@ -59,6 +59,8 @@ def _load() -> ctypes.CDLL:
         "orc_array_cholesky_solve": (None, [_i, _dp, _dp, _dp]),
         "orc_array_solve_nopivot": (None, [_i, _dp, _dp, _dp]),
         "orc_array_lu_nopivot": (None, [_i, _dp]),
+        "orc_sytf2_lower": (_i, [_i, _dp, _ip]),
+        "orc_ldl_unpack": (None, [_i, _dp, _ip, _dp, _dp, _vp]),
         "orc_array_inverse_nopivot": (None, [_i, _dp, _dp]),
         "orc_array_det": (ctypes.c_double, [_i, _dp, _i]),
         "orc_array_slogdet": (None, [_i, _dp, _dp, _dp]),
@@ -272,6 +274,27 @@ def array_slogdet_cholesky(lower) -> float:
     """``slogdet_cholesky``: ``2 sum ln L_ii``."""
     lower = _f64(lower)
     return float(_lib.orc_array_slogdet_cholesky(lower.shape[0], _p(lower)))
+
+
+def sytf2_lower(a):
+    """``sytf2_lower``: ``(packed factor, ipiv 1-based, info)``."""
+    f = _f64(a).copy()
+    n = f.shape[0]
+    ipiv = np.zeros(max(n, 1), dtype=np.int32)
+    info = _lib.orc_sytf2_lower(n, _p(f), ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return f, ipiv[:n], int(info)
+
+
+def ldl(a):
+    """numax's ``ldl``: SciPy's ``(lu, d, perm)``."""
+    f, ipiv, _ = sytf2_lower(a)
+    n = f.shape[0]
+    lu = np.zeros_like(f)
+    d = np.zeros_like(f)
+    perm = np.zeros(max(n, 1), dtype=np.int64)
+    _lib.orc_ldl_unpack(n, _p(f), ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(lu), _p(d),
+                        perm.ctypes.data_as(ctypes.c_void_p))
+    return lu, d, perm[:n]
 
 
 def array_eigh(a, sweeps: int = 12):

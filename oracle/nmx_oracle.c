@@ -772,6 +772,117 @@ double orc_array_slogdet_cholesky(int n, const double* l) {
   return total * 2.0;
 }
 
+/* sytf2_lower (numax/linalg/panel.mojo:2737-2894): Bunch-Kaufman L D L^T of
+ * the lower triangle in place, alpha = (1 + sqrt(17)) / 8, first-maximum
+ * scans, ipiv 1-based (kp + 1, or -(kp + 1) on both rows of a 2 x 2 block).
+ * Returns the first k + 1 whose column is exactly zero (0: none). */
+int orc_sytf2_lower(int n, double* a, int32_t* ipiv) {
+  const double alpha = 0.6403882032022076;
+  int info = 0;
+  int k = 0;
+#define A_(i, j) a[(int64_t)(i) * n + (j)]
+  while (k < n) {
+    int kstep = 1, kp = k, imax = k;
+    double absakk = fabs(A_(k, k)), colmax = 0.0;
+    for (int i = k + 1; i < n; i++) {
+      double v = fabs(A_(i, k));
+      if (v > colmax) { colmax = v; imax = i; }
+    }
+    if (absakk == 0.0 && colmax == 0.0) {
+      kp = k;
+      if (info == 0) info = k + 1;
+    } else if (absakk >= alpha * colmax) {
+      kp = k;
+    } else {
+      double rowmax = 0.0;
+      for (int j = k; j < imax; j++) { double v = fabs(A_(imax, j)); if (v > rowmax) rowmax = v; }
+      for (int i = imax + 1; i < n; i++) { double v = fabs(A_(i, imax)); if (v > rowmax) rowmax = v; }
+      if (absakk >= alpha * colmax * (colmax / rowmax)) kp = k;
+      else if (fabs(A_(imax, imax)) >= alpha * rowmax) kp = imax;
+      else { kp = imax; kstep = 2; }
+    }
+    if (kstep == 1) ipiv[k] = kp + 1;
+    else { ipiv[k] = -(kp + 1); ipiv[k + 1] = -(kp + 1); }
+    int kk = k + kstep - 1;
+    if (kp != kk) {
+      for (int i = kp + 1; i < n; i++) { double h = A_(i, kk); A_(i, kk) = A_(i, kp); A_(i, kp) = h; }
+      for (int j = kk + 1; j < kp; j++) { double h = A_(j, kk); A_(j, kk) = A_(kp, j); A_(kp, j) = h; }
+      double h = A_(kk, kk); A_(kk, kk) = A_(kp, kp); A_(kp, kp) = h;
+      if (kstep == 2) { double h2 = A_(k + 1, k); A_(k + 1, k) = A_(kp, k); A_(kp, k) = h2; }
+    }
+    if (kstep == 1) {
+      if (k < n - 1) {
+        double d11 = 1.0 / A_(k, k);
+        for (int j = k + 1; j < n; j++) {
+          double xj = A_(j, k);
+          for (int i = j; i < n; i++) A_(i, j) = A_(i, j) - d11 * A_(i, k) * xj;
+        }
+        for (int i = k + 1; i < n; i++) A_(i, k) = A_(i, k) * d11;
+      }
+    } else if (k < n - 2) {
+      double d21 = A_(k + 1, k);
+      double d11 = A_(k + 1, k + 1) / d21;
+      double d22 = A_(k, k) / d21;
+      double tt = 1.0 / (d11 * d22 - 1.0);
+      d21 = tt / d21;
+      for (int j = k + 2; j < n; j++) {
+        double wk = d21 * (d11 * A_(j, k) - A_(j, k + 1));
+        double wkp1 = d21 * (d22 * A_(j, k + 1) - A_(j, k));
+        for (int i = j; i < n; i++) A_(i, j) = A_(i, j) - A_(i, k) * wk - A_(i, k + 1) * wkp1;
+      }
+      for (int j = k + 2; j < n; j++) {
+        double wk = d21 * (d11 * A_(j, k) - A_(j, k + 1));
+        double wkp1 = d21 * (d22 * A_(j, k + 1) - A_(j, k));
+        A_(j, k) = wk;
+        A_(j, k + 1) = wkp1;
+      }
+    }
+    k += kstep;
+  }
+#undef A_
+  return info;
+}
+
+/* ldl_unpack_lower (numax/linalg/panel.mojo:2897-3003), SciPy's
+ * _ldl_sanitize_ipiv / _ldl_get_d_and_l / _ldl_construct_tri_factor:
+ * (lu, d, perm) from sytf2's packed factor. */
+void orc_ldl_unpack(int n, const double* ldu, const int32_t* ipiv, double* lu, double* d, int64_t* perm) {
+  int32_t* swap = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  int32_t* piv = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  int32_t* pr = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) { swap[i] = i; piv[i] = 0; pr[i] = i; }
+  int ind = 0;
+  while (ind < n) {
+    int cur = ipiv[ind];
+    if (cur > 0) { if (cur != ind + 1) swap[ind] = swap[cur - 1]; piv[ind] = 1; ind += 1; }
+    else { if (-cur != ind + 2) swap[ind + 1] = swap[-cur - 1]; piv[ind] = 2; ind += 2; }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      d[i * n + j] = i == j ? ldu[i * n + i] : 0.0;
+      lu[i * n + j] = i == j ? 1.0 : (i > j ? ldu[i * n + j] : 0.0);
+    }
+  for (int blk = 0; blk < n;) {
+    if (piv[blk] == 2) {
+      double v = ldu[(blk + 1) * n + blk];
+      d[(blk + 1) * n + blk] = v; d[blk * n + blk + 1] = v; lu[(blk + 1) * n + blk] = 0.0;
+      blk += 2;
+    } else {
+      blk += 1;
+    }
+  }
+  for (ind = n - 1; ind >= 0; ind--) {
+    int s = swap[ind];
+    if (s != ind) {
+      int col_s = piv[ind] == 0 ? ind - 1 : ind;
+      for (int c = col_s; c < n; c++) { double h = lu[s * n + c]; lu[s * n + c] = lu[ind * n + c]; lu[ind * n + c] = h; }
+      int32_t ps = pr[s]; pr[s] = pr[ind]; pr[ind] = ps;
+    }
+  }
+  for (int i = 0; i < n; i++) perm[pr[i]] = i;
+  free(swap); free(piv); free(pr);
+}
+
 /* _jacobi_rotation (_array/eigen.mojo:34-75). */
 static void jacobi_rotation(double num, double den, double* c, double* s) {
   double twice = 2.0 * num;

@@ -100,3 +100,17 @@ SLOGDET_SINGULAR = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [7.0, 8.0, 10.0]]
 # and general3 (leading minors nonzero, so the unpivoted LU is valid).
 SPD3 = np.array([[4.0, 2.0, 1.0], [2.0, 5.0, 3.0], [1.0, 3.0, 6.0]])
 GENERAL3 = np.array([[2.0, 1.0, -1.0], [-3.0, -1.0, 2.0], [-2.0, 1.0, 2.0]])
+
+# tests/linalg/test_ldl.mojo:25-111 -- SciPy's scipy.linalg.ldl of an
+# indefinite 4x4 (one 2 x 2 pivot block, interchanges) and of a 3x3 with an
+# all-zero diagonal.
+LDL_A = np.array([[1.0, 2.0, 3.0, 4.0], [2.0, -1.0, 0.5, 2.0], [3.0, 0.5, 0.2, -1.0], [4.0, 2.0, -1.0, 0.1]])
+LDL_LU = np.array([[1.0, 0.0, 0.0, 0.0], [0.49056603773584906, 0.37735849056603776, -0.3249656121045393, 1.0],
+                   [-0.27044025157232704, 0.8176100628930818, 1.0, 0.0], [0.0, 1.0, 0.0, 0.0]])
+LDL_D = np.array([[1.0, 4.0, 0.0, 0.0], [4.0, 0.1, 0.0, 0.0], [0.0, 0.0, 1.828930817610063, 0.0],
+                  [0.0, 0.0, 0.0, -2.9289889958734525]])
+LDL_PERM = np.array([0, 3, 2, 1])
+LDL_Z = np.array([[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [2.0, 3.0, 0.0]])
+LDL_Z_LU = np.array([[1.0, 0.0, 0.0], [1.5, 0.5, 1.0], [0.0, 1.0, 0.0]])
+LDL_Z_D = np.array([[0.0, 2.0, 0.0], [2.0, 0.0, 0.0], [0.0, 0.0, -3.0]])
+LDL_Z_PERM = np.array([0, 2, 1])

@@ -457,3 +457,45 @@ def test_array_inverse():
         # inverse = solve against each column of the identity
         np.testing.assert_array_equal(oracle.array_inverse(a)[:, n // 2],
                                       oracle.array_solve(a, np.eye(n)[:, n // 2]))
+
+
+# ----------------------------------------------------------------- ldl (sytf2)
+def test_ldl_known_answers():
+    """tests/linalg/test_ldl.mojo:49-111 (SciPy's factors)."""
+    lu, d, perm = oracle.ldl(kat.LDL_A)
+    np.testing.assert_allclose(lu, kat.LDL_LU, atol=1e-14)
+    np.testing.assert_allclose(d, kat.LDL_D, atol=1e-14)
+    np.testing.assert_array_equal(perm, kat.LDL_PERM)
+    lu, d, perm = oracle.ldl(kat.LDL_Z)
+    np.testing.assert_allclose(lu, kat.LDL_Z_LU, atol=1e-14)
+    np.testing.assert_allclose(d, kat.LDL_Z_D, atol=1e-14)
+    np.testing.assert_array_equal(perm, kat.LDL_Z_PERM)
+
+
+@pytest.mark.parametrize("n,seed", [(5, 1), (17, 2), (40, 3), (97, 4)])
+def test_ldl_matches_scipy(n, seed):
+    """SciPy's ldl calls LAPACK's blocked sytrf, whose pivot choices are
+    sytf2's: same permutation and block structure, factors to rounding."""
+    from scipy import linalg
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((n, n))
+    a = (g + g.T) / 2  # indefinite
+    lu, d, perm = oracle.ldl(a)
+    lu_s, d_s, perm_s = linalg.ldl(a, lower=True)
+    np.testing.assert_array_equal(perm, perm_s)
+    np.testing.assert_array_equal(d != 0, d_s != 0)  # the same 1 x 1 / 2 x 2 blocks
+    np.testing.assert_allclose(lu, lu_s, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(d, d_s, rtol=0, atol=1e-10 * np.abs(d_s).max())
+    np.testing.assert_allclose(lu @ d @ lu.T, a, rtol=0, atol=1e-11 * n)
+    # lu[perm] is unit lower triangular
+    t = lu[perm]
+    np.testing.assert_array_equal(np.diag(t), 1.0)
+    assert np.all(np.triu(t, 1) == 0.0)
+
+
+def test_ldl_definite_takes_only_scalar_pivots():
+    """tests/linalg/test_ldl.mojo:126-140."""
+    s = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.2], [0.5, 0.2, 2.0]])
+    lu, d, perm = oracle.ldl(s)
+    assert np.all(d == np.diag(np.diag(d)))
+    np.testing.assert_allclose(lu @ d @ lu.T, s, atol=1e-14)
