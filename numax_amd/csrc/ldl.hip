@@ -115,9 +115,13 @@ __global__ void __launch_bounds__(kLdlThreads) sytf2_lower_kernel(int n, T* __re
         kp = k;
       } else if (fabs(A_(imax, imax)) >= alpha * rowmax) {
         kp = imax;
-      } else {
+      } else if (k + 1 < n) {
         kp = imax;
         kstep = 2;
+      } else {
+        // the last column (imax == k): only NaN comparisons after a zero
+        // pivot get here, and a 2 x 2 block would start past the matrix
+        kp = k;
       }
     }
     __syncthreads();  // every thread has read what the decision needs before anything moves

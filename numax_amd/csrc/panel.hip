@@ -705,37 +705,58 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 template <typename T>
 __global__ void __launch_bounds__(256) potrf_diag_kernel(T* __restrict__ A, int lda, int k0, int nb,
                                                          int32_t* __restrict__ info) {
-  __shared__ T S[64][65];
+  // thread t: row i = t % 64 of the block, columns 16 q .. 16 q + 15 in
+  // registers (q = t / 64, one wavefront per column quarter).  Column j's
+  // owner wavefront scales it and publishes it through LDS (two buffers by
+  // parity); one barrier per column, then every row updates its own columns.
+  __shared__ T col[2][64];
   __shared__ int info_set;
-  const int t = threadIdx.x;
-  for (int idx = t; idx < nb * nb; idx += 256) {
-    int i = idx / nb, c = idx % nb;
-    if (c <= i) S[i][c] = A[(int64_t)(k0 + i) * lda + k0 + c];
+  const int t = threadIdx.x, i = t & 63, q = t >> 6;
+  T R[16];
+#pragma unroll
+  for (int cc = 0; cc < 16; cc++) {
+    const int c = 16 * q + cc;
+    R[cc] = (i < nb && c <= i) ? A[(int64_t)(k0 + i) * lda + k0 + c] : T(0);
   }
   if (t == 0) info_set = (*info != 0);
   __syncthreads();
-  for (int j = 0; j < nb; j++) {
-    if (t == 0) {
-      T d = S[j][j];
-      if (d <= T(0)) {
-        if (!info_set) { *info = k0 + j + 1; info_set = 1; }
-        d = T(kPivotFloor);
+  for (int qj = 0; qj < 4; qj++) {
+#pragma unroll
+    for (int cj = 0; cj < 16; cj++) {
+      const int j = 16 * qj + cj;
+      if (j >= nb) break;
+      T* cv = col[j & 1];
+      if (q == qj) {
+        const T d = __shfl(R[cj], j);  // the diagonal entry, from lane j
+        T dd = d;
+        if (dd <= T(0)) dd = T(kPivotFloor);
+        const T root = sqrt(dd);
+        if (i > j && i < nb) {
+          R[cj] = R[cj] / root;
+          cv[i] = R[cj];
+        } else if (i == j) {
+          R[cj] = root;
+          if (d <= T(0) && !info_set) {
+            *info = k0 + j + 1;
+            info_set = 1;
+          }
+        }
       }
-      S[j][j] = sqrt(d);
+      __syncthreads();
+      if (i > j && i < nb) {
+        const T lij = cv[i];
+#pragma unroll
+        for (int cc = 0; cc < 16; cc++) {
+          const int c = 16 * q + cc;
+          if (c > j && c <= i) R[cc] = R[cc] - lij * cv[c];
+        }
+      }
     }
-    __syncthreads();
-    const T root = S[j][j];
-    for (int i = j + 1 + t; i < nb; i += 256) S[i][j] = S[i][j] / root;
-    __syncthreads();
-    for (int idx = t; idx < nb * nb; idx += 256) {
-      int i = idx / nb, c = idx % nb;
-      if (i > j && c > j && c <= i) S[i][c] = S[i][c] - S[i][j] * S[c][j];
-    }
-    __syncthreads();
   }
-  for (int idx = t; idx < nb * nb; idx += 256) {
-    int i = idx / nb, c = idx % nb;
-    if (c <= i) A[(int64_t)(k0 + i) * lda + k0 + c] = S[i][c];
+#pragma unroll
+  for (int cc = 0; cc < 16; cc++) {
+    const int c = 16 * q + cc;
+    if (i < nb && c <= i) A[(int64_t)(k0 + i) * lda + k0 + c] = R[cc];
   }
 }
 

@@ -799,7 +799,12 @@ int orc_sytf2_lower(int n, double* a, int32_t* ipiv) {
       for (int i = imax + 1; i < n; i++) { double v = fabs(A_(i, imax)); if (v > rowmax) rowmax = v; }
       if (absakk >= alpha * colmax * (colmax / rowmax)) kp = k;
       else if (fabs(A_(imax, imax)) >= alpha * rowmax) kp = imax;
-      else { kp = imax; kstep = 2; }
+      /* A 2 x 2 block needs k + 1 < n.  At the last column (imax == k) only
+       * the NaN comparisons that follow a zero pivot reach this branch; the
+       * reference then stores ipiv[n], one past its end (panel.mojo:2809-2812),
+       * and here the step stays 1 x 1 instead. */
+      else if (k + 1 < n) { kp = imax; kstep = 2; }
+      else kp = k;
     }
     if (kstep == 1) ipiv[k] = kp + 1;
     else { ipiv[k] = -(kp + 1); ipiv[k + 1] = -(kp + 1); }

@@ -62,5 +62,9 @@ def test_ldl_singular_reports_info():
     a[1:, 1:] = [[2.0, 1.0, 0.0], [1.0, 3.0, 1.0], [0.0, 1.0, 4.0]]
     f = nl.ldl(a)
     assert f.info == 1
-    _, _, info = oracle.sytf2_lower(a)
+    _, ipo, info = oracle.sytf2_lower(a)
     assert info == 1
+    # after the zero pivot the trailing entries are NaN; no 2 x 2 block may
+    # start at the last column (it would store one entry past ipiv)
+    np.testing.assert_array_equal(f.ipiv.cpu().numpy(), ipo)
+    assert ipo[-1] > 0 or ipo[-2] == ipo[-1] < 0

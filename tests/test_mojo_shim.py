@@ -30,5 +30,5 @@ def test_every_entry_point_has_a_wrapper_with_the_headers_parameters():
         got = [a.split(":")[0].strip() for a in wrappers[name].split(",") if a.strip()]
         assert got == [gen.pname(p) for p, _ in params], name
         # the function type handed to get_function has one Mojo type per C parameter
-        sig = re.search(r'get_function\[fn \((.*?)\) -> [A-Za-z\[\]0-9]+\]\("' + name + '"\)', text).group(1)
+        sig = re.search(r'get_function\[fn \((.*?)\) -> [A-Za-z\[\]0-9]+\]\("' + re.escape(name) + r'"\)', text).group(1)
         assert len([t for t in re.split(r",\s*(?![^\[]*\])", sig) if t.strip()]) == len(params), name

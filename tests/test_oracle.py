@@ -493,6 +493,26 @@ def test_ldl_matches_scipy(n, seed):
     assert np.all(np.triu(t, 1) == 0.0)
 
 
+def test_sytf2_singular_stays_inside_ipiv():
+    """A zero first column: info = 1, the NaNs that follow the zero pivot
+    never open a 2 x 2 block at the last column, and nothing is stored past
+    ipiv's n entries (guard words after them keep their value)."""
+    import ctypes
+    for n in (2, 3, 4, 7):
+        a = np.zeros((n, n))
+        if n > 1:
+            g = np.arange(1.0, n) + 1.0
+            a[1:, 1:] = np.diag(g) + 0.5
+        f = np.ascontiguousarray(a)
+        ipiv = np.full(n + 4, -777, dtype=np.int32)
+        info = oracle._lib.orc_sytf2_lower(n, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        assert info == 1
+        np.testing.assert_array_equal(ipiv[n:], -777)
+        assert ipiv[n - 1] > 0 or (n >= 2 and ipiv[n - 2] == ipiv[n - 1] < 0)
+        assert np.all(np.abs(ipiv[:n]) >= 1) and np.all(np.abs(ipiv[:n]) <= n)
+
+
 def test_ldl_definite_takes_only_scalar_pivots():
     """tests/linalg/test_ldl.mojo:126-140."""
     s = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.2], [0.5, 0.2, 2.0]])
