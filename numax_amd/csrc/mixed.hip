@@ -1009,16 +1009,18 @@ static int mp_getrf_lookahead(nmx_ctx* ctx, MpFactor& f) {
   NMX_TRY(zero_info(ctx, f.info));
   bool behind = false;  // an update is queued on the second stream
   for (int k = 0; k < n; k += bw) {
-    const int nb = std::min(bw, n - k);
-    NMX_TRY(tslu_panel_launch(ctx, f.ts, f.ts_bytes, n, F, n, k, nb, f.ipiv, f.info, 0));
-    NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, 0, k));
+    const int nb = std::min(bw, n - k), slot = (k / bw) & 1;
+    NMX_TRY(tslu_panel_launch(ctx, f.ts, f.ts_bytes, n, F, n, k, nb, f.ipiv, f.info, 0, slot));
     if (behind) {
       NMX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->behind_ev, 0));
       behind = false;
     }
-    NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, k + nb, n));
+    NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, k + nb, n, slot));
     const int m = n - k - nb;
-    if (m <= 0) break;
+    if (m <= 0) {
+      NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, 0, k, slot));
+      break;
+    }
     NMX_TRY(trsm_left<float>(ctx, 0, 1, 0, nb, m, F + (int64_t)k * n + k, n, F + (int64_t)k * n + k + nb, n));
     NMX_TRY((mp_convert<BF, SPLIT>(ctx, f, k, nb, m)));
     const int near = std::min(bw, m);
@@ -1027,11 +1029,18 @@ static int mp_getrf_lookahead(nmx_ctx* ctx, MpFactor& f) {
       NMX_HIP(ctx, hipEventRecord(ctx->ahead_ev, ctx->stream));
       NMX_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->ahead_ev, 0));
       {
+        // the interchanges of the columns left of the panel feed nothing
+        // until the end: they trail on the second stream too, from this
+        // panel's record slot, which the panel after next refills only once
+        // the main stream has waited for this work
         StreamSwap on_aux(ctx, ctx->aux);
         NMX_TRY((mp_gemm_cols<BF, SPLIT>(ctx, f, k, nb, m, near, m)));
+        NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, 0, k, slot));
       }
       NMX_HIP(ctx, hipEventRecord(ctx->behind_ev, ctx->aux));
       behind = true;
+    } else {
+      NMX_TRY(tslu_swap_columns(ctx, f.ts, F, n, k, nb, 0, k, slot));
     }
   }
   if (behind) NMX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->behind_ev, 0));

@@ -193,10 +193,14 @@ int getrf_impl_ex(nmx_ctx* ctx, int n, T* A, int lda, int32_t* ipiv, int32_t* in
 // scratch of tslu_scratch_bytes(n), cleared once by tslu_scratch_init.
 size_t tslu_scratch_bytes(int n);
 int tslu_scratch_init(nmx_ctx* ctx, void* scratch);
+// A panel's interchange records go to record slot `slot` (0 or 1); a slot
+// keeps them until the panel after next.
 int tslu_panel_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int k, int nb,
-                      int32_t* ipiv, int32_t* info, int swap_outside);
-// The last panel's interchanges over columns c_lo .. c_hi outside it.
-int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi);
+                      int32_t* ipiv, int32_t* info, int swap_outside, int slot = 0);
+// The interchanges of the panel at k (records in `slot`) over columns
+// c_lo .. c_hi outside it.
+int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi,
+                      int slot = 0);
 
 template <typename T>
 int potrf_impl(nmx_ctx* ctx, int n, T* A, int lda, int32_t* info, int block);

@@ -600,25 +600,31 @@ __global__ void __launch_bounds__(64) tslu_swap_outside_kernel(float* __restrict
 
 }  // namespace
 
-// Scratch of the tournament: merge counters (zero between leaves), the
-// final merge's output, the candidate sets of every level.
+// Scratch of the tournament: merge counters (zero between leaves), two
+// slots of final-merge output (a panel's interchange records stay readable
+// while the next panel fills the other slot), the candidate sets of every
+// level.
 constexpr int kTsluMaxCounters = 4096;
 constexpr size_t kOutsBytes = ((sizeof(TsluOut) * kTsluMaxLeaves + 255) & ~(size_t)255);
 size_t tslu_scratch_bytes(int n) {
   const int G = ceil_div(n, kTsRows);
   size_t sets = 0;
   for (int s = G; s >= 1; s = s == 1 ? 0 : next_count(s)) sets += (size_t)s;
-  return 4 * (size_t)kTsluMaxCounters + kOutsBytes + 4 * sets * kTsW + 256;
+  return 4 * (size_t)kTsluMaxCounters + 2 * kOutsBytes + 4 * sets * kTsW + 256;
 }
 
 // One panel (columns k .. k+nb, rows k .. n) as a flat sequence of 32-column
 // leaves: tournament, apply (which also moves the panel's other columns and
 // forms U12), and one GEMM for the panel's columns right of the leaf; then
 // (swap_outside) the panel's interchanges over every column outside it.
-int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi) {
+static TsluOut* outs_slot(void* scratch, int slot) {
+  return (TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters + (slot & 1) * kOutsBytes);
+}
+
+int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int nb, int c_lo, int c_hi, int slot) {
   if (c_hi <= c_lo || nb <= 0) return NMX_OK;
   if (c_lo < k + nb && c_hi > k) return NMX_ERR_UNSUPPORTED;  // the panel's own columns are moved by the apply
-  const TsluOut* outs = (const TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters);
+  const TsluOut* outs = outs_slot(scratch, slot);
   ProfScope ps(ctx, K_LASWP, 0.0, 16.0 * nb * (double)(c_hi - c_lo));
   tslu_swap_outside_kernel<<<ceil_div(c_hi - c_lo, 64), 64, 0, ctx->stream>>>(A, lda, k, nb, c_lo, c_hi, outs,
                                                                                 ceil_div(nb, kTsW));
@@ -627,13 +633,13 @@ int tslu_swap_columns(nmx_ctx* ctx, void* scratch, float* A, int lda, int k, int
 }
 
 int tslu_panel_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, float* A, int lda, int k, int nb,
-                      int32_t* ipiv, int32_t* info, int swap_outside) {
+                      int32_t* ipiv, int32_t* info, int swap_outside, int slot) {
   if (nb <= 0 || n - k <= 0) return NMX_OK;
   if (tslu_scratch_bytes(n) > scratch_bytes) return NMX_ERR_UNSUPPORTED;
   if (ceil_div(nb, kTsW) > kTsluMaxLeaves) return NMX_ERR_UNSUPPORTED;
   int32_t* counters = (int32_t*)scratch;
-  TsluOut* outs = (TsluOut*)((char*)scratch + 4 * (size_t)kTsluMaxCounters);
-  int32_t* cand = (int32_t*)((char*)outs + kOutsBytes);
+  TsluOut* outs = outs_slot(scratch, slot);
+  int32_t* cand = (int32_t*)((char*)scratch + 4 * (size_t)kTsluMaxCounters + 2 * kOutsBytes);
   const int pc0 = k, pc1 = k + nb;
   for (int c0 = k; c0 < pc1; c0 += kTsW) {
     TsluOut* out = outs + (c0 - k) / kTsW;
@@ -676,8 +682,8 @@ int tslu_panel_launch(nmx_ctx* ctx, void* scratch, size_t scratch_bytes, int n, 
                                  0));
   }
   if (swap_outside) {
-    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, 0, k));
-    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, k + nb, n));
+    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, 0, k, slot));
+    NMX_TRY(tslu_swap_columns(ctx, scratch, A, lda, k, nb, k + nb, n, slot));
   }
   return NMX_OK;
 }
