@@ -566,6 +566,16 @@ __global__ void __launch_bounds__(256) tri_inv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 constexpr long kTrsvSpin = 1L << 23;
 
+// The pause between two polls of a block's published words, longer the
+// further the poller is from that block in the solve order: only the next
+// block's wait is on the chain's critical path, and the many later blocks
+// waiting for the same words would otherwise keep their few cache lines
+// busy with uncached reads.
+__device__ __forceinline__ void poll_pause(int distance) {
+  __builtin_amdgcn_s_sleep(1);
+  for (int z = 1; z < min(distance, 32); z++) __builtin_amdgcn_s_sleep(4);
+}
+
 // A 128 x 128 tile as two register slabs per lane: with 8 wavefronts, row
 // r = wave + 8 q (q < 16), columns lane and lane + 64 (coalesced row reads).
 // All 32 loads of a lane are in flight at once; rows >= rows or columns
@@ -686,7 +696,7 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
           wv = __hip_atomic_load(&xpub[j0 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if ((uint32_t)wv == epoch) break;
           if (aborted || ++spins > kTrsvSpin) { aborted = 1; break; }
-          __builtin_amdgcn_s_sleep(1);
+          poll_pause(steps - st);
         }
         v = __uint_as_float((uint32_t)(wv >> 32));
       }
@@ -732,6 +742,190 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
   finish(Iv);
   if (t == 0 && aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
   if (stamps && t == 0) stamps[bi] = __builtin_amdgcn_s_memrealtime();  // 100 MHz clock
+}
+
+// ---------------------------------------------------------------------------
+// The same blocked solve for a block of right-hand sides, Y = op^-1 T: one
+// workgroup per (128-row block, group of 16 NS right-hand sides), dispatched
+// block-major in solve order, so again a workgroup only waits for
+// workgroups dispatched before it; each group is its own chain.  Tiles are
+// loaded coalesced and staged through each wavefront's own LDS rows; the
+// products with the finished blocks and with the diagonal inverse run on
+// v_mfma_f32_16x16x4_f32, NS 16-column accumulators per wavefront sharing
+// each fragment.  Each solved entry is published as one 64-bit word (value
+// bits, epoch) in pub (ld ldp).
+// ---------------------------------------------------------------------------
+constexpr int kChainR = 16, kChainThreads = 512;
+constexpr int kChainLd = TB + 4;  // LDS row stride of a wavefront's staged tile rows
+constexpr int kChainGroup = TB * 16;
+typedef float chain_f4 __attribute__((ext_vector_type(4)));
+
+// A 128 x 128 tile in registers, loaded coalesced: instruction j of lane l
+// of wavefront w reads row 16 w + j / 2, column 64 (j & 1) + l; rows >= rows
+// or columns >= cols read 0.
+struct ChainTile {
+  float v[32];
+};
+__device__ __forceinline__ void chain_load(ChainTile& Tl, const float* __restrict__ tile, int64_t ld, int rows,
+                                           int cols, int wave, int lane) {
+  const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tile);
+  uint32_t vb = ((uint32_t)(16 * wave) * (uint32_t)ld + (uint32_t)lane) * 4u;
+  asm volatile("" : "+v"(vb));  // recomputed per tile, not hoisted out of the caller's loop
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int r = 16 * wave + j / 2, c = 64 * (j & 1) + lane;
+    const uint32_t off = vb + ((uint32_t)(j / 2) * (uint32_t)ld + 64u * (uint32_t)(j & 1)) * 4u;
+    Tl.v[j] = buf_load<float>(rt, (r < rows && c < cols) ? off : kBufOob);
+  }
+}
+// The wavefront's 16 rows into its own LDS rows (stride kChainLd): only this
+// wavefront reads them, after its own writes, so no barrier.
+__device__ __forceinline__ void chain_stage(const ChainTile& Tl, float* wl, int lane) {
+#pragma unroll
+  for (int j = 0; j < 32; j++) wl[(j / 2) * kChainLd + 64 * (j & 1) + lane] = Tl.v[j];
+}
+
+// acc[g] += (the staged 16 rows) . X_g for the 128 x 16 column groups X_g of a
+// staged block, row-major at xs + g * kChainGroup.  MFMA step s: lane l gives
+// row l & 15 at K index 4 s + (l >> 4) (bank-conflict-free with the row
+// stride kChainLd) and X_g's K index 4 s + (l >> 4), column l & 15.  Lane l,
+// item e of acc[g] is row 16 w + 4 (l >> 4) + e, column l & 15 of group g.
+__device__ __forceinline__ int chain_slot(int k, int c) { return (c >> 4) * kChainGroup + k * 16 + (c & 15); }
+template <int NS>
+__device__ __forceinline__ void chain_mma(const float* wl, const float* xs, int lane, chain_f4 (&acc)[NS]) {
+  const float* arow = wl + (lane & 15) * kChainLd + (lane >> 4);
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+    const float a = arow[4 * s];
+#pragma unroll
+    for (int g = 0; g < NS; g++)
+      acc[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xs[g * kChainGroup + 64 * s + lane], acc[g], 0, 0, 0);
+  }
+}
+
+template <int UPPER, int NS>
+__global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nrhs, const float* __restrict__ F,
+                                                                   int ldf, const float* __restrict__ inv,
+                                                                   const float* __restrict__ T, int ldt,
+                                                                   float* __restrict__ Y, int ldy,
+                                                                   uint64_t* __restrict__ pub, int ldp,
+                                                                   uint32_t epoch, int* __restrict__ err,
+                                                                   uint64_t* __restrict__ stamps) {
+  constexpr int CR = 16 * NS;                 // right-hand sides per workgroup
+  constexpr int WPT = TB * CR / kChainThreads;  // published words each thread takes per block
+  __shared__ float xs[2][NS * kChainGroup];
+  __shared__ float wtile[kChainThreads / 64][16 * kChainLd];
+  __shared__ int aborted;
+  const int nblk = (n + TB - 1) / TB, groups = (nrhs + CR - 1) / CR;
+  const int step = (int)blockIdx.x / groups, grp = (int)blockIdx.x % groups;
+  const int bi = UPPER ? nblk - 1 - step : step;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int k0 = bi * TB, nb = min(TB, n - k0), c0 = grp * CR, nc = min(CR, nrhs - c0);
+  if (t == 0) aborted = 0;
+  if (stamps && t == 0) stamps[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();  // 100 MHz clock
+  // this block's right-hand sides and diagonal inverse depend on no other
+  // block: loaded first
+  const int rbase = 16 * wave + 4 * (lane >> 4), cl = lane & 15;
+  float tv[NS][4];
+#pragma unroll
+  for (int g = 0; g < NS; g++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int r = rbase + e, c = 16 * g + cl;
+      tv[g][e] = (r < nb && c < nc) ? T[(int64_t)(k0 + r) * ldt + c0 + c] : 0.f;
+    }
+  ChainTile Fa, Fb, Fi;
+  float* wl = wtile[wave];
+  chain_load(Fi, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
+  auto col_block = [&](int st) { return UPPER ? nblk - 1 - st : st; };
+  auto fetch = [&](ChainTile& Tl, int st) {
+    const int j0 = col_block(st) * TB;
+    chain_load(Tl, F + (int64_t)k0 * ldf + j0, ldf, nb, min(TB, n - j0), wave, lane);
+  };
+  // wait for block st's solution in this group (WPT consecutive words per
+  // thread, all polled together), staged in xs[st & 1]
+  auto take = [&](int st) {
+    const int j0 = col_block(st) * TB, jb = min(TB, n - j0);
+    const int k = t / (CR / WPT), cb = (t % (CR / WPT)) * WPT;
+    const uint64_t* src = pub + (int64_t)(j0 + k) * ldp + c0 + cb;
+    // every word is loaded on every pass, an unwanted one from pub[0], so
+    // that all the loads are in flight before the one wait
+    bool want[WPT];
+    const uint64_t* at[WPT];
+#pragma unroll
+    for (int i = 0; i < WPT; i++) {
+      want[i] = k < jb && cb + i < nc;
+      at[i] = want[i] ? src + i : pub;
+    }
+    uint64_t wv[WPT];
+    long spins = 0;
+    for (;;) {
+#pragma unroll
+      for (int i = 0; i < WPT; i++) wv[i] = __hip_atomic_load(at[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool ok = true;
+#pragma unroll
+      for (int i = 0; i < WPT; i++) ok = ok && (!want[i] || (uint32_t)wv[i] == epoch);
+      if (ok) break;
+      if (aborted || ++spins > kTrsvSpin) {
+        aborted = 1;
+        break;
+      }
+      poll_pause(step - st);
+    }
+    float* dst = xs[st & 1];
+#pragma unroll
+    for (int i = 0; i < WPT; i++) dst[chain_slot(k, cb + i)] = want[i] ? __uint_as_float((uint32_t)(wv[i] >> 32)) : 0.f;
+    __syncthreads();
+  };
+  __syncthreads();
+  const int steps = step;
+  chain_f4 acc[NS];
+#pragma unroll
+  for (int g = 0; g < NS; g++) acc[g] = chain_f4{0.f, 0.f, 0.f, 0.f};
+  if (steps > 0) fetch(Fa, 0);
+  int st = 0;
+  // each tile is staged before the wait for the block it multiplies
+  for (; st + 1 < steps; st += 2) {
+    fetch(Fb, st + 1);
+    chain_stage(Fa, wl, lane);
+    take(st);
+    chain_mma<NS>(wl, xs[st & 1], lane, acc);
+    if (st + 2 < steps) fetch(Fa, st + 2);
+    chain_stage(Fb, wl, lane);
+    take(st + 1);
+    chain_mma<NS>(wl, xs[(st + 1) & 1], lane, acc);
+  }
+  if (st < steps) {
+    chain_stage(Fa, wl, lane);
+    take(st);
+    chain_mma<NS>(wl, xs[st & 1], lane, acc);
+  }
+  // rhs = T - (the products), staged in the buffer no thread still reads
+  // (the last take used the other one); then x = inverse . rhs
+  float* rb = xs[steps & 1];
+  chain_stage(Fi, wl, lane);
+#pragma unroll
+  for (int g = 0; g < NS; g++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) rb[chain_slot(rbase + e, 16 * g + cl)] = tv[g][e] - acc[g][e];
+  __syncthreads();
+  chain_f4 x[NS];
+#pragma unroll
+  for (int g = 0; g < NS; g++) x[g] = chain_f4{0.f, 0.f, 0.f, 0.f};
+  chain_mma<NS>(wl, rb, lane, x);
+#pragma unroll
+  for (int g = 0; g < NS; g++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int r = rbase + e, c = 16 * g + cl;
+      if (r < nb && c < nc) {
+        Y[(int64_t)(k0 + r) * ldy + c0 + c] = x[g][e];
+        __hip_atomic_store(&pub[(int64_t)(k0 + r) * ldp + c0 + c], ((uint64_t)__float_as_uint(x[g][e]) << 32) | epoch,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  if (t == 0 && aborted) atomicExch(err, NMX_ERR_SYNC_TIMEOUT);
+  if (stamps && t == 0) stamps[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
 }
 
 // ---------------------------------------------------------------------------
@@ -838,9 +1032,11 @@ struct MpFactor {
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
   unsigned* maxbits = nullptr;       // [2] per step
   uint64_t* xpub = nullptr;          // TRSV published solution words (n)
+  uint64_t* mpub = nullptr;          // block-solve published words (n x mpub_ld), when set up
+  int mpub_ld = 0;
   void* ts = nullptr;                // tournament leaf scratch
   size_t ts_bytes = 0;
-  uint32_t epoch = 0;
+  uint32_t epoch = 0, mepoch = 0;
   bool ok = false;
 };
 
@@ -857,7 +1053,9 @@ struct MpBuffers {
   void* lo = nullptr;
   unsigned* maxbits = nullptr;
   uint64_t* xpub = nullptr;
-  uint32_t epoch = 0;
+  uint64_t* mpub = nullptr;
+  size_t mpub_elems = 0;
+  uint32_t epoch = 0, mepoch = 0;
 };
 
 static void mp_buffers_free(void* p) {
@@ -871,6 +1069,7 @@ static void mp_buffers_free(void* p) {
   hipFree(b->lo);
   hipFree(b->maxbits);
   hipFree(b->xpub);
+  hipFree(b->mpub);
   hipFree(b->ts);
   delete b;
 }
@@ -937,10 +1136,32 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   return NMX_OK;
 }
 
-// Hands the TRSV epoch back to the cache (flags keep their last values).
+// The published words of the block solves for nrhs right-hand sides (after
+// mp_alloc): n x round_up(nrhs, kChainR), cleared when they grow.
+static int mp_alloc_chains(nmx_ctx* ctx, MpFactor& f, int nrhs) {
+  MpBuffers* b = (MpBuffers*)ctx->mp_cache;
+  const int ld = ceil_div(nrhs, kChainR) * kChainR;
+  const size_t want = (size_t)f.n * ld;
+  if (want > b->mpub_elems) {
+    NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    NMX_TRY(grow(ctx, b->mpub, b->mpub_elems, want, sizeof(uint64_t)));
+    NMX_HIP(ctx, hipMemsetAsync(b->mpub, 0, sizeof(uint64_t) * want, ctx->stream));
+    b->mepoch = 0;
+  }
+  f.mpub = b->mpub;
+  f.mpub_ld = ld;
+  f.mepoch = b->mepoch;
+  return NMX_OK;
+}
+
+// Hands the solve epochs back to the cache (published words keep their last
+// values).
 static void mp_free(nmx_ctx* ctx, MpFactor& f) {
   MpBuffers* b = (MpBuffers*)ctx->mp_cache;
-  if (b) b->epoch = f.epoch;
+  if (b) {
+    b->epoch = f.epoch;
+    if (f.mpub) b->mepoch = f.mepoch;
+  }
   f = MpFactor();
 }
 
@@ -1268,14 +1489,33 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   return NMX_OK;
 }
 
-// Y := op^-1 T for nrhs right-hand sides (n x nrhs, ld nrhs; T is used as
-// scratch), op = L (unit lower) or U, right-looking over the 128-row blocks
-// of the factorization: each diagonal block is solved by its precomputed
-// inverse (one GEMM), then the blocks still to solve are updated with one
-// GEMM -- two MFMA launches per block, none of them a long dependent chain.
+// Y := op^-1 T for nrhs right-hand sides (n x nrhs, ld nrhs), op = L (unit
+// lower) or U.  With the published words set up (mp_alloc_chains): one
+// trsm_chain_kernel launch, T only read.  Otherwise right-looking over the
+// 128-row blocks (T used as scratch): each diagonal block solved by its
+// precomputed inverse (one GEMM), then the blocks still to solve updated with
+// one GEMM -- two MFMA launches per block.
 static int mp_trsm_blocks(nmx_ctx* ctx, MpFactor& f, int upper, int nrhs, float* T, float* Y) {
   const int n = f.n, nblk = ceil_div(n, TB);
   const float* inv = upper ? f.invU : f.invL;
+  if (f.mpub && nrhs <= f.mpub_ld) {
+    // one launch: the chains of trsm_chain_kernel (T is only read)
+    f.mepoch += 1;
+    ProfScope ps(ctx, K_TRSM, (double)n * n * nrhs, 4.0 * ((double)n * n / 2 + 2.0 * n * nrhs));
+    // one 16-column group per workgroup: a hop's two 128 x 128 x 16 products
+    // already take a CU's FP32 MFMA rate for about 1.7 us, so wider groups
+    // lengthen the chain (measured: 32 or 64 columns per workgroup 1.5x and
+    // 3x slower at 64 right-hand sides); the groups run as separate chains
+    const int grid = nblk * ceil_div(nrhs, kChainR);
+    if (upper)
+      trsm_chain_kernel<1, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, T, nrhs, Y, nrhs, f.mpub,
+                                                                        f.mpub_ld, f.mepoch, ctx->err, nullptr);
+    else
+      trsm_chain_kernel<0, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, T, nrhs, Y, nrhs, f.mpub,
+                                                                        f.mpub_ld, f.mepoch, ctx->err, nullptr);
+    NMX_HIP(ctx, hipGetLastError());
+    return NMX_OK;
+  }
   for (int s = 0; s < nblk; s++) {
     const int kb = upper ? nblk - 1 - s : s;
     const int r0 = kb * TB, bs = std::min(TB, n - r0);
@@ -1587,6 +1827,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   f.pivot = cfg.pivot;
   int st = mp_alloc(ctx, f, n, cfg.block, cfg.precision);
   f.pivot = cfg.pivot;
+  if (st == NMX_OK) st = mp_alloc_chains(ctx, f, nrhs);
   void* wsp = nullptr;
   const size_t blk = (size_t)n * nrhs;
   if (st == NMX_OK)

@@ -6,6 +6,7 @@
 //   nmx_profile lu    N [NRHS]   FP64 getrf + getrs, A ~ N(0,1), NRHS (1) right-hand sides
 //   nmx_profile chol  N [NRHS]   FP64 potrf + potrs, A = G G^T / n + I, NRHS (1) right-hand sides
 //   nmx_profile mpir  N [PREC]   mixed-precision LU-IR, A ~ N(0,1) + 0 I
+//   nmx_profile mprhs N [NRHS]   the same for NRHS (64) right-hand sides
 //   nmx_profile batch COUNT      uneven batch, n ~ U{8..256}, half LU / half
 //                                Cholesky, FP32 factor + FP64 refinement
 //   nmx_profile eigh  COUNT      batched Jacobi eigh, n in {24, 32, 48, 64} in equal
@@ -202,6 +203,28 @@ static int run_mpir(nmx_ctx* ctx, int n, int prec) {
   return 0;
 }
 
+static int run_mprhs(nmx_ctx* ctx, int n, int nrhs) {
+  double *A, *B, *X;
+  HCHECK(hipMalloc(&A, sizeof(double) * n * n));
+  HCHECK(hipMalloc(&B, sizeof(double) * n * nrhs));
+  HCHECK(hipMalloc(&X, sizeof(double) * n * nrhs));
+  CHECK(nmx_fill_random_f64(ctx, 1, 7, 0, (int64_t)n * n, 0, 0, A));
+  CHECK(nmx_fill_random_f64(ctx, 1, 8, 0, (int64_t)n * nrhs, 0, 0, B));
+  nmx_ir_cfg cfg;
+  nmx_ir_cfg_default(&cfg);
+  std::vector<nmx_ir_report> reps(nrhs);
+  for (int it = 0; it < 2; it++) {
+    CHECK(nmx_gesv_mpir_nrhs(ctx, n, nrhs, A, n, B, nrhs, X, nrhs, &cfg, reps.data()));
+    CHECK(nmx_sync(ctx));
+    int steps = 0;
+    for (auto& r : reps) steps = std::max(steps, r.iterations);
+    printf("mprhs step %d: n=%d nrhs=%d iterations %d factor %.2f ms refine %.2f ms\n", it, n, nrhs, steps,
+           reps[0].factor_ms, reps[0].refine_ms);
+    fflush(stdout);
+  }
+  return 0;
+}
+
 static int run_batch(nmx_ctx* ctx, int count) {
   std::mt19937_64 rng(11);
   std::uniform_int_distribution<int> un(8, 256);
@@ -273,7 +296,7 @@ static int run_batch(nmx_ctx* ctx, int count) {
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: nmx_profile copy BYTES | lu N [NRHS] | chol N [NRHS] | mpir N [PREC] | batch COUNT | "
+    fprintf(stderr, "usage: nmx_profile copy BYTES | lu N [NRHS] | chol N [NRHS] | mpir N [PREC] | mprhs N [NRHS] | batch COUNT | "
                     "eigh COUNT | lstsq COUNT\n");
     return 2;
   }
@@ -287,6 +310,7 @@ int main(int argc, char** argv) {
   else if (!strcmp(mode, "eigh")) rc = run_eigh(ctx, atoi(argv[2]));
   else if (!strcmp(mode, "lstsq")) rc = run_lstsq(ctx, atoi(argv[2]));
   else if (!strcmp(mode, "mpir")) rc = run_mpir(ctx, atoi(argv[2]), argc > 3 ? atoi(argv[3]) : 2);
+  else if (!strcmp(mode, "mprhs")) rc = run_mprhs(ctx, atoi(argv[2]), argc > 3 ? atoi(argv[3]) : 64);
   else if (!strcmp(mode, "batch")) rc = run_batch(ctx, atoi(argv[2]));
   else fprintf(stderr, "unknown mode %s\n", mode);
   CHECK(nmx_ctx_destroy(ctx));

@@ -140,11 +140,15 @@ def test_gmres_ir_odd_sizes(n):
     assert rep["converged"] and eta <= 1e-15, (rep, eta)
 
 
-@pytest.mark.parametrize("nrhs,kappa,pivot", [(3, 1e4, "partial"), (64, 1e6, "partial"), (5, 1e2, "butterfly")])
+@pytest.mark.parametrize("nrhs,kappa,pivot", [(3, 1e4, "partial"), (64, 1e6, "partial"), (5, 1e2, "butterfly"),
+                                              (20, 1e4, "tournament"), (100, 1e2, "tournament")])
 def test_mpir_block_of_right_hand_sides(nrhs, kappa, pivot):
     """One factorization, every column refined to FP64 backward error; each
-    column agrees with its single-right-hand-side solve to what kappa allows."""
-    n = 1001 if pivot == "partial" else 512
+    column agrees with its single-right-hand-side solve to what kappa allows.
+    The block solves take up to 64 right-hand sides per workgroup in groups
+    of 16: 3, 5, 20 (a partial second group), 64 and 100 (two workgroups per
+    block, the second partial) cover every shape."""
+    n = 512 if pivot == "butterfly" else 1001
     a = svd_matrix(n, kappa, seed=5)
     b = R.normal(6, (n, nrhs))
     x, reps = nl.solve_mpir(a, b, precision="fp16s", pivot=pivot)
