@@ -55,7 +55,7 @@ constexpr int BM = 128, BK = 16, PAD = 4;  // the large tile, K slab, LDS row pa
 template <typename T, int TA, int TB, int TILE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T beta,
-            T* __restrict__ C, int ldc, int tilesM, int tilesN, int lower_only) {
+            T* __restrict__ C, int ldc, int tilesM, int tilesN, int lower_only, int ksplit, int kchunk, int64_t csplit) {
   typedef typename Mfma<T>::acc_t acc_t;
   // TILE x TILE of C per workgroup; each wavefront owns WT x WT = FR x FR
   // MFMA tiles; each thread stages LPT elements of each K slab
@@ -64,9 +64,19 @@ gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda, cons
   __shared__ T As[2][BK][TM + PAD];
   __shared__ T Bs[2][BK][TN + PAD];
 
+  // ---- split K: slice q of the grid (a multiple of kXcds workgroups) takes
+  // K range [q kchunk, (q + 1) kchunk) into its own C at C + q csplit ----
+  const int per_slice = gridDim.x / ksplit, q = blockIdx.x / per_slice;
+  {
+    const int k_lo = q * kchunk;
+    K = max(0, min(kchunk, K - k_lo));
+    A += TA ? (int64_t)k_lo * lda : k_lo;
+    B += TB ? k_lo : (int64_t)k_lo * ldb;
+    C += q * csplit;
+  }
   // ---- tile assignment: contiguous chunk per XCD, grouped rows inside ----
   const int total = lower_only ? tilesM * (tilesM + 1) / 2 : tilesM * tilesN;
-  const int b = blockIdx.x;
+  const int b = blockIdx.x - q * per_slice;
   const int per = (total + kXcds - 1) / kXcds;
   const int lin = (b % kXcds) * per + b / kXcds;
   if (lin >= total) return;
@@ -276,16 +286,16 @@ int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alp
     constexpr int TL = decltype(tile_tag)::value;
     if (!transa && !transb)
       gemm_kernel<T, 0, 0, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
-                                                            lower_only);
+                                                            lower_only, 1, k, 0);
     else if (!transa && transb)
       gemm_kernel<T, 0, 1, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
-                                                            lower_only);
+                                                            lower_only, 1, k, 0);
     else if (transa && !transb)
       gemm_kernel<T, 1, 0, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
-                                                            lower_only);
+                                                            lower_only, 1, k, 0);
     else
       gemm_kernel<T, 1, 1, TL><<<g, blk, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tilesM, tilesN,
-                                                            lower_only);
+                                                            lower_only, 1, k, 0);
   };
   if (tile == 64)
     run(std::integral_constant<int, 64>());
@@ -294,6 +304,29 @@ int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alp
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
+
+// C_q = alpha A[:, K_q] B[K_q, :] for the ksplit K slices K_q of length
+// kchunk = round_up(ceil(k / ksplit), 16) (C_q at C + q csplit, no beta):
+// one launch with ksplit times the workgroups, for products whose output
+// alone would leave the device under-filled (n x 64 residuals).  The caller
+// sums the C_q in a fixed order.
+template <typename T>
+int gemm_launch_splitk(nmx_ctx* ctx, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T* C,
+                       int ldc, int ksplit, int64_t csplit) {
+  if (m <= 0 || n <= 0 || k <= 0 || ksplit < 1) return NMX_ERR_UNSUPPORTED;
+  const int kchunk = ceil_div(ceil_div(k, ksplit), 16) * 16;
+  constexpr int TL = 64;
+  const int tilesM = ceil_div(m, TL), tilesN = ceil_div(n, TL);
+  const int per_slice = ceil_div(tilesM * tilesN, kXcds) * kXcds;
+  ProfScope ps(ctx, K_GEMM, 2.0 * m * (double)n * k,
+               (double)sizeof(T) * ((double)m * k + (double)k * n + (double)ksplit * m * n));
+  gemm_kernel<T, 0, 0, TL><<<per_slice * ksplit, 256, 0, ctx->stream>>>(m, n, k, alpha, A, lda, B, ldb, T(0), C, ldc,
+                                                                         tilesM, tilesN, 0, ksplit, kchunk, csplit);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+template int gemm_launch_splitk<double>(nmx_ctx*, int, int, int, double, const double*, int, const double*, int,
+                                        double*, int, int, int64_t);
 
 template int gemm_launch<double>(nmx_ctx*, int, int, int, int, int, double, const double*, int, const double*, int,
                                  double, double*, int, int);

@@ -1373,6 +1373,20 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
 }
 
 // ---- multi-RHS refinement building blocks -------------------------------------
+// R = B - (P_0 + ... + P_{kResSplit-1}) over an n x nrhs block, the split-K
+// partial products of A X summed in index order
+constexpr int kResSplit = 4;
+__global__ void residual_combine_kernel(int n, int nrhs, const double* __restrict__ B, int ldb,
+                                        const double* __restrict__ P, double* __restrict__ R) {
+  const int64_t total = (int64_t)n * nrhs, idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int r = (int)(idx / nrhs), c = (int)(idx % nrhs);
+  double s = P[idx];
+#pragma unroll
+  for (int q = 1; q < kResSplit; q++) s += P[q * total + idx];
+  R[idx] = B[(int64_t)r * ldb + c] - s;
+}
+
 // x (+)= y over an n x nrhs block (row-major, leading dimensions ldy / ldx)
 __global__ void mat_f32_add_f64(int n, int nrhs, const float* __restrict__ y, int ldy, double* __restrict__ x,
                                 int ldx, int accumulate) {
@@ -1831,8 +1845,8 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   void* wsp = nullptr;
   const size_t blk = (size_t)n * nrhs;
   if (st == NMX_OK)
-    st = nmx_workspace(ctx, sizeof(double) * blk + sizeof(unsigned long long) * (2 * (size_t)nrhs + 8) +
-                                sizeof(int32_t) * (size_t)nrhs + 2 * sizeof(float) * blk + 512,
+    st = nmx_workspace(ctx, sizeof(double) * blk * (1 + kResSplit) + sizeof(unsigned long long) * (2 * (size_t)nrhs + 8) +
+                                sizeof(int32_t) * (size_t)nrhs + 2 * sizeof(float) * blk + 1024,
                        &wsp);
   if (st != NMX_OK) {
     mp_free(ctx, f);
@@ -1843,6 +1857,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   int32_t* flag = (int32_t*)(norms + 2 * (size_t)nrhs + 8);
   float* T = (float*)(((uintptr_t)(flag + nrhs) + 255) & ~(uintptr_t)255);
   float* Yb = (float*)(((uintptr_t)(T + blk) + 255) & ~(uintptr_t)255);
+  double* P = (double*)(((uintptr_t)(Yb + blk) + 255) & ~(uintptr_t)255);  // kResSplit partial products
 
   hipEvent_t e0, e1, e2;
   hipEventCreate(&e0);
@@ -1864,19 +1879,24 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
     memcpy(&anorm, &ha, 8);
   }
   std::vector<unsigned long long> hn(2 * (size_t)nrhs);
-  std::vector<int32_t> hflag(nrhs, 0);
+  // two host copies of the column flags, used alternately: the copy queued
+  // at step it is not rewritten before the norms read-back of step it + 1
+  // has synchronized past it
+  std::vector<int32_t> hflag_ring(2 * (size_t)nrhs, 0);
   std::vector<double> eta(nrhs, INFINITY), prev(nrhs, INFINITY);
   std::vector<int> done(nrhs, 0);
   int it = 0;
   while (st == NMX_OK) {
-    // R = B - A X in FP64 (MFMA GEMM), then the per-column norms
-    NMX_HIP(ctx, hipMemcpy2DAsync(R, sizeof(double) * nrhs, B, sizeof(double) * ldb, sizeof(double) * nrhs, n,
-                                  hipMemcpyDeviceToDevice, ctx->stream));
+    // R = B - A X in FP64: the MFMA GEMM split over K (an n x nrhs output
+    // alone is one workgroup per 64 rows), the partials summed in order;
+    // then the per-column norms
     {
       ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n * nrhs, 8.0 * (double)n * n + 24.0 * (double)n * nrhs);
-      st = gemm_launch<double>(ctx, 0, 0, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, nrhs);
+      st = gemm_launch_splitk<double>(ctx, n, nrhs, n, 1.0, A, lda, X, ldx, P, nrhs, kResSplit, (int64_t)blk);
+      if (st != NMX_OK) break;
+      residual_combine_kernel<<<(unsigned)((blk + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, B, ldb, P, R);
+      NMX_HIP(ctx, hipGetLastError());
     }
-    if (st != NMX_OK) break;
     {
       const int z = 2 * nrhs;
       for (int o = 0; o < z; o += 64) zero_u64<<<1, 64, 0, ctx->stream>>>(norms + o, std::min(64, z - o));
@@ -1903,15 +1923,15 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
         prev[c] = eta[c];
         active++;
       }
-      hflag[c] = done[c];
     }
+    for (int c = 0; c < nrhs; c++) hflag_ring[(size_t)(it & 1) * nrhs + c] = done[c];
     if (!active) break;
-    NMX_HIP(ctx, hipMemcpyAsync(flag, hflag.data(), sizeof(int32_t) * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    NMX_HIP(ctx, hipMemcpyAsync(flag, hflag_ring.data() + (size_t)(it & 1) * nrhs, sizeof(int32_t) * nrhs,
+                                hipMemcpyHostToDevice, ctx->stream));
     zero_cols_kernel<<<(unsigned)((blk + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, R, nrhs, flag);
     NMX_HIP(ctx, hipGetLastError());
     st = mp_solve_mat(ctx, f, nrhs, R, nrhs, X, ldx, T, Yb, 1);
     it++;
-    NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // hflag's host memory is rewritten next step
   }
   hipEventRecord(e2, ctx->stream);
   hipEventSynchronize(e2);

@@ -148,6 +148,10 @@ int nmx_epoch_reserve(nmx_ctx* ctx, uint32_t columns, uint32_t* base);
 template <typename T>
 int gemm_launch(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, T alpha, const T* A, int lda, const T* B,
                 int ldb, T beta, T* C, int ldc, int lower_only = 0);
+// ksplit K slices of A B (no transposes) into ksplit outputs C + q csplit
+template <typename T>
+int gemm_launch_splitk(nmx_ctx* ctx, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T* C,
+                       int ldc, int ksplit, int64_t csplit);
 
 // Left-side triangular solve op(T) X = B (vectors are columns of B).
 template <typename T>
