@@ -189,8 +189,21 @@ typedef struct nmx_ir_report {
 } nmx_ir_report;
 
 void nmx_ir_cfg_default(nmx_ir_cfg* cfg);
+/* Classical refinement (solver 1, and solver 0 until a step stops halving
+ * the backward error) runs as a fixed launch sequence whose stop test is
+ * made on the device, so the call can be captured between nmx_graph_begin
+ * and nmx_graph_end once it has run eagerly on the context (buffers sized,
+ * second stream created); a captured call fills no report (it returns
+ * NMX_OK) and holds max_iter + 1 steps, the ones past the stop returning at
+ * once.  Under capture solver 0 means classical refinement only; solver 2
+ * (GMRES-IR, decided on the host) and pivot 1 (the cooperative
+ * partial-pivoting panel, host-side epochs) return NMX_ERR_UNSUPPORTED. */
 int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x, const nmx_ir_cfg* cfg,
                   nmx_ir_report* report);
+/* The report of the last nmx_gesv_mpir run on the context's stream, eager
+ * or replayed from a graph (synchronizes; the timings and GMRES count are
+ * not kept on the device and read 0).  No reference counterpart. */
+int nmx_ir_report_read(nmx_ctx* ctx, nmx_ir_report* report);
 /* Autotuner of the panel width (no reference counterpart): times the
  * factorization of a synthetic N(0,1) matrix of order n at block 128, 256
  * and 512 and writes the fastest into cfg->block (other fields as given). */
@@ -405,9 +418,10 @@ int nmx_tensor_file_save(nmx_ctx* ctx, const char* path, int format, int dtype, 
  * the batched solvers (whose refinement loops run inside their kernels),
  * the batched eigensolver and least squares, QR, GEMM, TRSM.  A call that
  * would need more workspace than an earlier call left (run it once first),
- * the pivoting LU drivers (their cooperative panels take host-side epochs)
- * and the mixed-precision drivers (their loops read norms on the host)
- * return NMX_ERR_UNSUPPORTED while capturing.  A live graph keeps every
+ * the pivoting LU drivers (their cooperative panels take host-side epochs),
+ * the block-of-right-hand-sides and GMRES-IR mixed-precision solves (their
+ * loops read norms on the host) return NMX_ERR_UNSUPPORTED while capturing;
+ * nmx_gesv_mpir with classical refinement is captured whole.  A live graph keeps every
  * context buffer it reads: a workspace that grows afterwards gets a new
  * buffer and the old one is freed with the context's last graph.  No
  * reference counterpart: for callers that replay the same batch shape

@@ -103,6 +103,7 @@ SIGNATURES = {
                                           _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "nmx_ir_cfg_default": (None, [_c_vp]),
     "nmx_gesv_mpir": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "nmx_ir_report_read": (_c_int, [_c_vp, _c_vp]),
     "nmx_mp_factor_debug": (_c_int, [_c_vp, _c_int, _c_vp, _c_int, _c_int, _c_int, _c_u64, _c_int, _c_int,
                                      _c_vp, _c_vp, _c_vp, _c_vp]),
 }

@@ -172,7 +172,9 @@ __global__ void rbt_vec_t_kernel(int n, int nrhs, const double* __restrict__ v, 
 
 // x (FP64) += V y (y FP32), per column.
 __global__ void rbt_vec_n_add_kernel(int n, int nrhs, const float* __restrict__ y, int ldy, double* __restrict__ x,
-                                     int ldx, const float* __restrict__ dv, int accumulate) {
+                                     int ldx, const float* __restrict__ dv, int accumulate,
+                                     const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const int n4 = n & ~3, h = n4 / 4;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t groups = (int64_t)h * nrhs;
@@ -650,8 +652,14 @@ __device__ __forceinline__ float wave_reduce(float s[N], int lane) {
 template <int UPPER>
 __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const float* __restrict__ F, int ldf,
                                                            const float* __restrict__ inv, float* __restrict__ x,
-                                                           uint64_t* __restrict__ xpub, uint32_t epoch,
-                                                           int* __restrict__ err, uint64_t* __restrict__ stamps) {
+                                                           uint64_t* __restrict__ xpub,
+                                                           const uint32_t* __restrict__ ebase, uint32_t eoff,
+                                                           const int* __restrict__ stop, int* __restrict__ err,
+                                                           uint64_t* __restrict__ stamps) {
+  // a stopped refinement skips the whole solve (every workgroup returns, so
+  // none waits); the epoch of this launch is the device base plus its offset
+  if (stop && *stop) return;
+  const uint32_t epoch = *ebase + eoff;
   // Blocks are solved in dispatch order (lower: first block first; upper:
   // last block first), and a block only waits for blocks dispatched before
   // it, so the chain needs no co-residency guarantee.  Each solved entry is
@@ -745,6 +753,46 @@ __global__ void __launch_bounds__(kTrsvThreads) trsv_blocked_kernel(int n, const
 }
 
 // ---------------------------------------------------------------------------
+// M_i = inv_i . op_(i, i -+ 1): block i's diagonal inverse times the tile
+// that couples it to the block solved just before it (lower: left of the
+// diagonal, upper: right of it); zero for the first block solved.  With it
+// a block solve x_i = inv_i (t_i - sum_j op_ij x_j) needs, once x of that
+// previous block arrives, one product: x_i = pre_i - M_i x_(i -+ 1).
+// ---------------------------------------------------------------------------
+template <int UPPER>
+__global__ void __launch_bounds__(512) couple_kernel(int n, const float* __restrict__ F, int ldf,
+                                                     const float* __restrict__ inv, float* __restrict__ out) {
+  __shared__ float Iv[TB][TB + 1];
+  __shared__ float Tt[TB][TB + 1];
+  const int nblk = (n + TB - 1) / TB, bi = blockIdx.x, t = threadIdx.x;
+  const int k0 = bi * TB, nb = min(TB, n - k0), bj = UPPER ? bi + 1 : bi - 1;
+  float* o = out + (int64_t)bi * TB * TB;
+  if (bj < 0 || bj >= nblk) {
+    for (int idx = t; idx < TB * TB; idx += 512) o[idx] = 0.f;
+    return;
+  }
+  const int j0 = bj * TB, jb = min(TB, n - j0);
+  const float* iv = inv + (int64_t)bi * TB * TB;
+  for (int idx = t; idx < TB * TB; idx += 512) {
+    const int r = idx / TB, c = idx % TB;
+    Iv[r][c] = (r < nb && c < nb) ? iv[idx] : 0.f;
+    Tt[r][c] = (r < nb && c < jb) ? F[(int64_t)(k0 + r) * ldf + j0 + c] : 0.f;
+  }
+  __syncthreads();
+  const int r = t >> 2, c0 = (t & 3) * 32;
+  float acc[32];
+#pragma unroll
+  for (int cc = 0; cc < 32; cc++) acc[cc] = 0.f;
+  for (int k = 0; k < TB; k++) {
+    const float a = Iv[r][k];
+#pragma unroll
+    for (int cc = 0; cc < 32; cc++) acc[cc] += a * Tt[k][c0 + cc];
+  }
+#pragma unroll
+  for (int cc = 0; cc < 32; cc++) o[r * TB + c0 + cc] = acc[cc];
+}
+
+// ---------------------------------------------------------------------------
 // The same blocked solve for a block of right-hand sides, Y = op^-1 T: one
 // workgroup per (128-row block, group of 16 NS right-hand sides), dispatched
 // block-major in solve order, so again a workgroup only waits for
@@ -806,6 +854,7 @@ __device__ __forceinline__ void chain_mma(const float* wl, const float* xs, int 
 template <int UPPER, int NS>
 __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nrhs, const float* __restrict__ F,
                                                                    int ldf, const float* __restrict__ inv,
+                                                                   const float* __restrict__ cpl,
                                                                    const float* __restrict__ T, int ldt,
                                                                    float* __restrict__ Y, int ldy,
                                                                    uint64_t* __restrict__ pub, int ldp,
@@ -813,7 +862,7 @@ __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nr
                                                                    uint64_t* __restrict__ stamps) {
   constexpr int CR = 16 * NS;                 // right-hand sides per workgroup
   constexpr int WPT = TB * CR / kChainThreads;  // published words each thread takes per block
-  __shared__ float xs[2][NS * kChainGroup];
+  __shared__ float xs[3][NS * kChainGroup];  // two for the waits, one for the block's own right-hand side
   __shared__ float wtile[kChainThreads / 64][16 * kChainLd];
   __shared__ int aborted;
   const int nblk = (n + TB - 1) / TB, groups = (nrhs + CR - 1) / CR;
@@ -834,9 +883,10 @@ __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nr
       const int r = rbase + e, c = 16 * g + cl;
       tv[g][e] = (r < nb && c < nc) ? T[(int64_t)(k0 + r) * ldt + c0 + c] : 0.f;
     }
-  ChainTile Fa, Fb, Fi;
+  ChainTile Fa, Fb, Fi, Fm;
   float* wl = wtile[wave];
   chain_load(Fi, inv + (int64_t)bi * TB * TB, TB, nb, nb, wave, lane);
+  chain_load(Fm, cpl + (int64_t)bi * TB * TB, TB, nb, TB, wave, lane);
   auto col_block = [&](int st) { return UPPER ? nblk - 1 - st : st; };
   auto fetch = [&](ChainTile& Tl, int st) {
     const int j0 = col_block(st) * TB;
@@ -878,7 +928,9 @@ __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nr
     __syncthreads();
   };
   __syncthreads();
-  const int steps = step;
+  // every block solved before this one except the last: its product is
+  // folded into the coupling tile Fm = inv . op_(bi, last)
+  const int steps = step > 0 ? step - 1 : 0;
   chain_f4 acc[NS];
 #pragma unroll
   for (int g = 0; g < NS; g++) acc[g] = chain_f4{0.f, 0.f, 0.f, 0.f};
@@ -900,9 +952,9 @@ __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nr
     take(st);
     chain_mma<NS>(wl, xs[st & 1], lane, acc);
   }
-  // rhs = T - (the products), staged in the buffer no thread still reads
-  // (the last take used the other one); then x = inverse . rhs
-  float* rb = xs[steps & 1];
+  // pre = inverse . (T - the products), the right-hand side staged in its
+  // own buffer; then, once the last block arrives, x = pre - Fm . x_last
+  float* rb = xs[2];
   chain_stage(Fi, wl, lane);
 #pragma unroll
   for (int g = 0; g < NS; g++)
@@ -913,6 +965,16 @@ __global__ void __launch_bounds__(kChainThreads) trsm_chain_kernel(int n, int nr
 #pragma unroll
   for (int g = 0; g < NS; g++) x[g] = chain_f4{0.f, 0.f, 0.f, 0.f};
   chain_mma<NS>(wl, rb, lane, x);
+  if (step > 0) {
+    chain_stage(Fm, wl, lane);
+    take(steps);  // the block solved just before this one
+    chain_f4 cm[NS];
+#pragma unroll
+    for (int g = 0; g < NS; g++) cm[g] = chain_f4{0.f, 0.f, 0.f, 0.f};
+    chain_mma<NS>(wl, xs[steps & 1], lane, cm);
+#pragma unroll
+    for (int g = 0; g < NS; g++) x[g] -= cm[g];
+  }
 #pragma unroll
   for (int g = 0; g < NS; g++)
 #pragma unroll
@@ -938,7 +1000,9 @@ constexpr int kResidualChunks = 16;
 __global__ void __launch_bounds__(256) residual_kernel(int n, const double* __restrict__ A, int lda,
                                                        const double* __restrict__ x, const double* __restrict__ b,
                                                        double* __restrict__ r, double* __restrict__ rowabs,
-                                                       unsigned long long* __restrict__ norms) {
+                                                       unsigned long long* __restrict__ norms,
+                                                       const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ unsigned long long red[3][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x);
@@ -1008,7 +1072,62 @@ __global__ void cast_f64_f32(int n, const double* __restrict__ A, int lda, float
   F[(int64_t)r * ldf + c] = (float)A[(int64_t)r * lda + c];
 }
 
-__global__ void vec_f32_add_f64(int n, const float* __restrict__ y, double* __restrict__ x, int accumulate) {
+// Device-side state of one classical refinement: its steps run as a fixed
+// launch sequence (a graph can hold it), each kernel returning at once when
+// `stop` is set; ir_decide_kernel makes the host loop's decision per step.
+struct IrDev {
+  int32_t stop, it, converged, pad;
+  double eta, prev, anorm;
+  double history[32];
+  uint32_t ebase, pad2;  // TRSV epoch base: each solve moves it by kEpochSpan
+};
+constexpr uint32_t kEpochSpan = 1u << 20;  // TRSV launches one solve may make
+
+__global__ void ir_init_kernel(IrDev* s) {
+  if (threadIdx.x == 0) {
+    *s = IrDev{};
+    s->ebase = 0;
+  }
+}
+__global__ void ir_epoch_bump_kernel(IrDev* s) {
+  if (threadIdx.x == 0) s->ebase += kEpochSpan;
+}
+__global__ void ir_reset_kernel(IrDev* s) {
+  if (threadIdx.x == 0) {
+    s->stop = 0;
+    s->it = 0;
+    s->converged = 0;
+    s->eta = __builtin_inf();
+    s->prev = __builtin_inf();
+    s->anorm = 0.0;
+    for (int i = 0; i < 32; i++) s->history[i] = 0.0;
+    s->ebase += kEpochSpan;
+  }
+}
+// After a residual: eta = max|r| / (||A||_inf max|x|) (first: ||A||_inf from
+// the residual's row sums), recorded; stop when converged, out of steps or
+// no longer halving; otherwise count the correction about to be applied.
+__global__ void ir_decide_kernel(IrDev* s, const unsigned long long* __restrict__ norms, double tol, int max_iter,
+                                 int first) {
+  if (threadIdx.x != 0 || s->stop) return;
+  const double rm = __longlong_as_double((long long)norms[0]), xm = __longlong_as_double((long long)norms[1]);
+  if (first) s->anorm = __longlong_as_double((long long)norms[2]);
+  const double eta = rm / (s->anorm * xm);
+  const int it = s->it;
+  s->eta = eta;
+  s->history[it < 32 ? it : 31] = eta;
+  if (!(eta > tol) || it >= max_iter || (it > 0 && !(eta < 0.5 * s->prev))) {
+    s->stop = 1;
+    s->converged = eta <= tol ? 1 : 0;
+  } else {
+    s->prev = eta;
+    s->it = it + 1;
+  }
+}
+
+__global__ void vec_f32_add_f64(int n, const float* __restrict__ y, double* __restrict__ x, int accumulate,
+                                const int* __restrict__ stop) {
+  if (stop && *stop) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = accumulate ? x[i] + (double)y[i] : (double)y[i];
 }
@@ -1032,11 +1151,15 @@ struct MpFactor {
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
   unsigned* maxbits = nullptr;       // [2] per step
   uint64_t* xpub = nullptr;          // TRSV published solution words (n)
+  IrDev* ird = nullptr;              // device-side refinement state and TRSV epoch base
+  uint32_t eoff = 0;                 // TRSV epoch offset of the next launch in this solve
   uint64_t* mpub = nullptr;          // block-solve published words (n x mpub_ld), when set up
   int mpub_ld = 0;
+  float* cplL = nullptr;             // block-solve coupling tiles (couple_kernel), when set up
+  float* cplU = nullptr;
   void* ts = nullptr;                // tournament leaf scratch
   size_t ts_bytes = 0;
-  uint32_t epoch = 0, mepoch = 0;
+  uint32_t mepoch = 0;
   bool ok = false;
 };
 
@@ -1053,9 +1176,13 @@ struct MpBuffers {
   void* lo = nullptr;
   unsigned* maxbits = nullptr;
   uint64_t* xpub = nullptr;
+  IrDev* ird = nullptr;
+  size_t ird_elems = 0;
   uint64_t* mpub = nullptr;
   size_t mpub_elems = 0;
-  uint32_t epoch = 0, mepoch = 0;
+  float* cpl = nullptr;
+  size_t cpl_elems = 0;
+  uint32_t mepoch = 0;
 };
 
 static void mp_buffers_free(void* p) {
@@ -1069,15 +1196,26 @@ static void mp_buffers_free(void* p) {
   hipFree(b->lo);
   hipFree(b->maxbits);
   hipFree(b->xpub);
+  hipFree(b->ird);
   hipFree(b->mpub);
+  hipFree(b->cpl);
   hipFree(b->ts);
   delete b;
 }
 
+// Grows a cached buffer.  Never under capture (run the call once first);
+// while captured graphs live, the old buffer is retired, not freed (they may
+// still read it), and released with the last graph.
 template <typename P>
 static int grow(nmx_ctx* ctx, P*& ptr, size_t& have, size_t want, size_t elem) {
   if (want <= have && ptr) return NMX_OK;
-  if (ptr) hipFree(ptr);
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;
+  if (ptr) {
+    if (ctx->live_graphs > 0)
+      ctx->retired.push_back((void*)ptr);
+    else
+      hipFree(ptr);
+  }
   ptr = nullptr;
   have = 0;
   if (hipMalloc((void**)&ptr, want * elem + 256) != hipSuccess) return NMX_ERR_ALLOC;
@@ -1095,7 +1233,7 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
     ctx->mp_cache = b;
     ctx->mp_cache_free = mp_buffers_free;
   }
-  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->capturing) NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const int nblk = ceil_div(n, TB);
   size_t lo_bytes = (size_t)n * block * 4 * 2;  // A_hi, A_lo, B_hi, B_lo (lo unused unless split)
   size_t unused = 0, unused2 = 0;
@@ -1110,8 +1248,14 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
     NMX_TRY(grow(ctx, b->xpub, b->pub_elems, (size_t)n, sizeof(uint64_t)));
     hipMemsetAsync(b->xpub, 0, sizeof(uint64_t) * (size_t)n, ctx->stream);
     NMX_HIP(ctx, hipGetLastError());
-    b->epoch = 0;
   }
+  if (!b->ird) {
+    NMX_TRY(grow(ctx, b->ird, b->ird_elems, 1, sizeof(IrDev)));
+    ir_init_kernel<<<1, 64, 0, ctx->stream>>>(b->ird);
+    NMX_HIP(ctx, hipGetLastError());
+  }
+  f.ird = b->ird;
+  f.eoff = 0;
   f.F = b->F;
   f.dw = b->d;
   f.dv = b->d + 2 * (size_t)n;
@@ -1123,7 +1267,6 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.lo_buf = b->lo;
   f.maxbits = b->maxbits;
   f.xpub = b->xpub;
-  f.epoch = b->epoch;
   if (f.pivot == 2) {
     const size_t want = tslu_scratch_bytes(n);
     if (want > b->ts_bytes) {
@@ -1151,17 +1294,27 @@ static int mp_alloc_chains(nmx_ctx* ctx, MpFactor& f, int nrhs) {
   f.mpub = b->mpub;
   f.mpub_ld = ld;
   f.mepoch = b->mepoch;
+  const size_t tiles = (size_t)ceil_div(f.n, TB) * TB * TB;
+  NMX_TRY(grow(ctx, b->cpl, b->cpl_elems, 2 * tiles, sizeof(float)));
+  f.cplL = b->cpl;
+  f.cplU = b->cpl + tiles;
   return NMX_OK;
 }
 
-// Hands the solve epochs back to the cache (published words keep their last
-// values).
+// The coupling tiles of both triangles, after the factorization.
+static int mp_couple(nmx_ctx* ctx, MpFactor& f) {
+  const int nblk = ceil_div(f.n, TB);
+  couple_kernel<0><<<nblk, 512, 0, ctx->stream>>>(f.n, f.F, f.n, f.invL, f.cplL);
+  couple_kernel<1><<<nblk, 512, 0, ctx->stream>>>(f.n, f.F, f.n, f.invU, f.cplU);
+  NMX_HIP(ctx, hipGetLastError());
+  return NMX_OK;
+}
+
+// Hands the block-solve epoch back to the cache (published words keep their
+// last values).
 static void mp_free(nmx_ctx* ctx, MpFactor& f) {
   MpBuffers* b = (MpBuffers*)ctx->mp_cache;
-  if (b) {
-    b->epoch = f.epoch;
-    if (f.mpub) b->mepoch = f.mepoch;
-  }
+  if (b && f.mpub) b->mepoch = f.mepoch;
   f = MpFactor();
 }
 
@@ -1474,7 +1627,10 @@ __global__ void gm_scale_kernel(int n, double alpha, const double* __restrict__ 
 }
 
 // d (FP64, n) = V U^-1 L^-1 W^T r  (accumulate: x += d)
-static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, float* tmp, int accumulate) {
+// x (+)= M^-1 r; with `stop` (device), every kernel returns at once once it
+// is set.
+static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, float* tmp, int accumulate,
+                        const int* stop = nullptr) {
   const int n = f.n, n4 = n & ~3, h = n4 / 4;
   const int64_t work = (int64_t)h + (n - n4);
   if (f.pivot) {
@@ -1485,19 +1641,26 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
   }
   const int nblk = ceil_div(n, TB);
   for (int up = 0; up < 2; up++) {
-    f.epoch += 1;
+    if (f.eoff + 1 >= kEpochSpan) {  // a very long solve: move the base on
+      ir_epoch_bump_kernel<<<1, 64, 0, ctx->stream>>>(f.ird);
+      f.eoff = 0;
+    }
+    f.eoff += 1;
+    const uint32_t* ebase = &f.ird->ebase;
     ProfScope ps(ctx, K_MP_TRSV, 2.0 * (double)n * n / 2, 4.0 * (double)n * n / 2);
     if (up)
-      trsv_blocked_kernel<1><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invU, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+      trsv_blocked_kernel<1><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invU, tmp, f.xpub, ebase, f.eoff,
+                                                                      stop, ctx->err, nullptr);
     else
-      trsv_blocked_kernel<0><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invL, tmp, f.xpub, f.epoch, ctx->err, nullptr);
+      trsv_blocked_kernel<0><<<nblk, kTrsvThreads, 0, ctx->stream>>>(n, f.F, n, f.invL, tmp, f.xpub, ebase, f.eoff,
+                                                                      stop, ctx->err, nullptr);
     NMX_HIP(ctx, hipGetLastError());
   }
   if (f.pivot) {
-    vec_f32_add_f64<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, tmp, x, accumulate);
+    vec_f32_add_f64<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(n, tmp, x, accumulate, stop);
   } else {
     rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, 1, tmp, 1, x, 1, f.dv,
-                                                                                   accumulate);
+                                                                                   accumulate, stop);
   }
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
@@ -1512,7 +1675,7 @@ static int mp_solve_vec(nmx_ctx* ctx, MpFactor& f, const double* r, double* x, f
 static int mp_trsm_blocks(nmx_ctx* ctx, MpFactor& f, int upper, int nrhs, float* T, float* Y) {
   const int n = f.n, nblk = ceil_div(n, TB);
   const float* inv = upper ? f.invU : f.invL;
-  if (f.mpub && nrhs <= f.mpub_ld) {
+  if (f.mpub && f.cplL && nrhs <= f.mpub_ld) {
     // one launch: the chains of trsm_chain_kernel (T is only read)
     f.mepoch += 1;
     ProfScope ps(ctx, K_TRSM, (double)n * n * nrhs, 4.0 * ((double)n * n / 2 + 2.0 * n * nrhs));
@@ -1522,11 +1685,11 @@ static int mp_trsm_blocks(nmx_ctx* ctx, MpFactor& f, int upper, int nrhs, float*
     // 3x slower at 64 right-hand sides); the groups run as separate chains
     const int grid = nblk * ceil_div(nrhs, kChainR);
     if (upper)
-      trsm_chain_kernel<1, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, T, nrhs, Y, nrhs, f.mpub,
-                                                                        f.mpub_ld, f.mepoch, ctx->err, nullptr);
+      trsm_chain_kernel<1, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, f.cplU, T, nrhs, Y, nrhs,
+                                                                        f.mpub, f.mpub_ld, f.mepoch, ctx->err, nullptr);
     else
-      trsm_chain_kernel<0, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, T, nrhs, Y, nrhs, f.mpub,
-                                                                        f.mpub_ld, f.mepoch, ctx->err, nullptr);
+      trsm_chain_kernel<0, 1><<<grid, kChainThreads, 0, ctx->stream>>>(n, nrhs, f.F, n, inv, f.cplL, T, nrhs, Y, nrhs,
+                                                                        f.mpub, f.mpub_ld, f.mepoch, ctx->err, nullptr);
     NMX_HIP(ctx, hipGetLastError());
     return NMX_OK;
   }
@@ -1566,7 +1729,7 @@ static int mp_solve_mat(nmx_ctx* ctx, MpFactor& f, int nrhs, const double* R, in
                                                                                           accumulate);
   } else {
     rbt_vec_n_add_kernel<<<(unsigned)((work + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, T, nrhs, D, ldd, f.dv,
-                                                                                  accumulate);
+                                                                                  accumulate, nullptr);
   }
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
@@ -1578,7 +1741,7 @@ static int residual(nmx_ctx* ctx, int n, const double* A, int lda, const double*
   {
     ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n + 24.0 * n);
     residual_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda, x, b, r, rowabs,
-                                                                                       norms);
+                                                                                       norms, nullptr);
     NMX_HIP(ctx, hipGetLastError());
   }
   unsigned long long h[3];
@@ -1629,7 +1792,8 @@ static int gmres_correction(nmx_ctx* ctx, MpFactor& f, const double* A, int lda,
     {
       ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n);
       residual_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda, vj, nullptr,
-                                                                                         g.w, nullptr, nullptr);
+                                                                                         g.w, nullptr, nullptr,
+                                                                                         nullptr);
       NMX_HIP(ctx, hipGetLastError());
     }
     NMX_TRY(mp_solve_vec(ctx, f, g.w, vn, tmp, 0));  // u = -M^-1 A v_j
@@ -1687,7 +1851,6 @@ static int gmres_correction(nmx_ctx* ctx, MpFactor& f, const double* A, int lda,
 NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const double* b, double* x,
                           const nmx_ir_cfg* cfg_in, nmx_ir_report* rep) {
   if (!ctx) return -1;
-  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;  // the refinement loop reads norms on the host
   if (n <= 0) return -2;
   if (!A) return -3;
   if (lda < n) return -4;
@@ -1704,9 +1867,14 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   if (cfg.block % 64) return -7;
   if (cfg.solver < 0 || cfg.solver > 2) return -7;
   if (cfg.solver != 1 && (cfg.gmres_restart < 1 || cfg.gmres_restart > 100)) return -7;
+  if (cfg.max_iter < 0) return -7;
   if (n > 65536) return NMX_ERR_UNSUPPORTED;
   if (cfg.pivot < 0 || cfg.pivot > 2) return -7;
   if (cfg.pivot == 1 && n > kLeafMaxRows) return NMX_ERR_UNSUPPORTED;  // checked before anything is allocated
+  // Under capture only classical refinement (its stop test runs on the
+  // device); GMRES-IR decides on the host.
+  const bool cap = ctx->capturing;
+  if (cap && (cfg.solver == 2 || cfg.pivot == 1)) return NMX_ERR_UNSUPPORTED;  // host decisions / host-side epochs
   nmx_ir_report local{};
   nmx_ir_report* R = rep ? rep : &local;
   *R = nmx_ir_report{};
@@ -1737,50 +1905,93 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   }
   float* tmp = (float*)((double*)(norms + 8) + gm_doubles);
 
-  hipEvent_t e0, e1, e2;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventCreate(&e2);
-  hipEventRecord(e0, ctx->stream);
-  st = mp_factor(ctx, f, A, lda, cfg.seed);
-  hipEventRecord(e1, ctx->stream);
-  if (st == NMX_OK) {
-    // x0 = M^-1 b
-    st = mp_solve_vec(ctx, f, b, x, tmp, 0);
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  if (!cap) {
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    hipEventRecord(e0, ctx->stream);
   }
-  double nrm[3] = {0, 0, 0};
-  double anorm = 0.0;
+  ir_reset_kernel<<<1, 64, 0, ctx->stream>>>(f.ird);  // also moves the TRSV epoch base on
+  NMX_HIP(ctx, hipGetLastError());
+  st = mp_factor(ctx, f, A, lda, cfg.seed);
+  if (!cap) hipEventRecord(e1, ctx->stream);
+  if (st == NMX_OK) st = mp_solve_vec(ctx, f, b, x, tmp, 0);  // x0 = M^-1 b
   int it = 0;
-  double eta = INFINITY, prev = INFINITY;
-  // solver 1: classical steps only; 2: GMRES-IR from the first correction;
-  // 0: classical while each step at least halves the backward error,
-  // GMRES-IR from the first step that does not (the preconditioner is
-  // applied in FP32, so where classical steps still contract GMRES costs
-  // more solves for the same accuracy)
-  bool use_gmres = cfg.solver == 2;
+  double eta = INFINITY, prev = INFINITY, anorm = 0.0;
   int inner_total = 0;
-  while (st == NMX_OK) {
-    st = residual(ctx, n, A, lda, x, b, r, it == 0 ? rowabs : nullptr, norms, nrm);
-    if (st != NMX_OK) break;
-    if (it == 0) anorm = nrm[2];
-    eta = nrm[0] / (anorm * nrm[1]);
-    R->history[it < 32 ? it : 31] = eta;
-    if (!(eta > cfg.tol) || it >= cfg.max_iter) break;
-    if (it > 0 && !(eta < 0.5 * prev)) {  // stagnation
-      if (cfg.solver != 0 || use_gmres) break;
-      use_gmres = true;
+  IrDev hs{};
+  bool gmres_phase = cfg.solver == 2;
+  if (st == NMX_OK && !gmres_phase) {
+    // Classical refinement as a fixed launch sequence: step s is the FP64
+    // residual, the device-side stop test, and (s < max_iter) the
+    // correction, every kernel returning at once after the stop.  Eagerly,
+    // the host looks at the stop word every kStepsPerCheck steps; a captured
+    // graph holds all max_iter + 1 steps.
+    constexpr int kStepsPerCheck = 4;
+    const int* stop = &f.ird->stop;
+    const int steps = cfg.max_iter + 1, chunk = cap ? steps : kStepsPerCheck;
+    for (int s0 = 0; s0 < steps && st == NMX_OK; s0 += chunk) {
+      for (int sk = s0; sk < std::min(s0 + chunk, steps) && st == NMX_OK; sk++) {
+        zero_u64<<<1, 64, 0, ctx->stream>>>(norms, 3);
+        {
+          ProfScope ps(ctx, K_RESIDUAL, 2.0 * (double)n * n, 8.0 * (double)n * n + 24.0 * n);
+          residual_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(
+              n, A, lda, x, b, r, sk == 0 ? rowabs : nullptr, norms, stop);
+        }
+        ir_decide_kernel<<<1, 64, 0, ctx->stream>>>(f.ird, norms, cfg.tol, cfg.max_iter, sk == 0);
+        NMX_HIP(ctx, hipGetLastError());
+        if (sk < cfg.max_iter) st = mp_solve_vec(ctx, f, r, x, tmp, 1, stop);
+      }
+      if (!cap && st == NMX_OK) {
+        int32_t hstop = 0;
+        NMX_HIP(ctx, hipMemcpyAsync(&hstop, stop, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hstop) break;
+      }
     }
-    prev = eta;
-    if (use_gmres) {
+    if (!cap && st == NMX_OK) {
+      NMX_HIP(ctx, hipMemcpyAsync(&hs, f.ird, sizeof(IrDev), hipMemcpyDeviceToHost, ctx->stream));
+      NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      it = hs.it;
+      eta = hs.eta;
+      prev = hs.prev;
+      anorm = hs.anorm;
+      for (int i = 0; i < 32; i++) R->history[i] = hs.history[i];
+      // auto: a classical run that stopped without converging and with steps
+      // left stagnated -- GMRES-IR takes over from here (solver 0: classical
+      // while each step at least halves the backward error; the
+      // preconditioner is applied in FP32, so where classical steps still
+      // contract GMRES costs more solves for the same accuracy)
+      if (cfg.solver == 0 && !hs.converged && it < cfg.max_iter && eta == eta) {
+        gmres_phase = true;
+        prev = INFINITY;
+      }
+    }
+  }
+  if (gmres_phase) {
+    double nrm[3] = {0, 0, 0};
+    while (st == NMX_OK) {
+      st = residual(ctx, n, A, lda, x, b, r, it == 0 && anorm == 0.0 ? rowabs : nullptr, norms, nrm);
+      if (st != NMX_OK) break;
+      if (anorm == 0.0) anorm = nrm[2];
+      eta = nrm[0] / (anorm * nrm[1]);
+      R->history[it < 32 ? it : 31] = eta;
+      if (!(eta > cfg.tol) || it >= cfg.max_iter) break;
+      if (it > 0 && !(eta < 0.5 * prev)) break;  // stagnation
+      prev = eta;
       // reduce the preconditioned residual by what the target still needs, with margin
       const double tol_in = std::min(1e-2, std::max(1e-10, 0.1 * cfg.tol / eta));
       int inner = 0;
       st = gmres_correction(ctx, f, A, lda, r, x, tmp, gm, tol_in, &inner);
       inner_total += inner;
-    } else {
-      st = mp_solve_vec(ctx, f, r, x, tmp, 1);
+      it++;
     }
-    it++;
+  }
+  if (cap) {
+    // the graph records the report on the device: nmx_ir_report_read after a replay
+    mp_free(ctx, f);
+    return st;
   }
   R->gmres_iterations = inner_total;
   hipEventRecord(e2, ctx->stream);
@@ -1804,6 +2015,26 @@ NMX_API int nmx_gesv_mpir(nmx_ctx* ctx, int n, const double* A, int lda, const d
   int dev = nmx_ctx_check(ctx);
   if (dev != NMX_OK) return dev;
   return R->converged ? NMX_OK : NMX_ERR_NOT_CONVERGED;
+}
+
+NMX_API int nmx_ir_report_read(nmx_ctx* ctx, nmx_ir_report* rep) {
+  if (!ctx) return -1;
+  if (!rep) return -2;
+  if (ctx->capturing) return NMX_ERR_UNSUPPORTED;
+  MpBuffers* b = (MpBuffers*)ctx->mp_cache;
+  if (!b || !b->ird || !b->info) return NMX_ERR_UNSUPPORTED;  // no mixed-precision solve has run on this context
+  IrDev hs{};
+  int32_t hinfo = 0;
+  NMX_HIP(ctx, hipMemcpyAsync(&hs, b->ird, sizeof(IrDev), hipMemcpyDeviceToHost, ctx->stream));
+  NMX_HIP(ctx, hipMemcpyAsync(&hinfo, b->info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *rep = nmx_ir_report{};
+  rep->iterations = hs.it;
+  rep->converged = hs.converged;
+  rep->backward_error = hs.eta;
+  rep->info = hinfo;
+  for (int i = 0; i < 32; i++) rep->history[i] = hs.history[i];
+  return nmx_ctx_check(ctx);
 }
 
 NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, int lda, const double* B, int ldb,
@@ -1865,6 +2096,7 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   hipEventCreate(&e2);
   hipEventRecord(e0, ctx->stream);
   st = mp_factor(ctx, f, A, lda, cfg.seed);
+  if (st == NMX_OK) st = mp_couple(ctx, f);
   hipEventRecord(e1, ctx->stream);
   if (st == NMX_OK) st = mp_solve_mat(ctx, f, nrhs, B, ldb, X, ldx, T, Yb, 0);  // X0 = M^-1 B
   double anorm = 0.0;

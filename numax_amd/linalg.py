@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["LU", "PLU", "LDL", "NumaxError", "lu_factor", "lu", "ldl", "inverse", "solve", "solve_mpir", "cholesky", "cholesky_solve", "solve_triangular",
+__all__ = ["LU", "PLU", "LDL", "NumaxError", "lu_factor", "lu", "ldl", "inverse", "solve", "solve_mpir", "solve_mpir_graph", "cholesky", "cholesky_solve", "solve_triangular",
            "gemm", "PRECISIONS"]
 
 # trailing-update precisions of the mixed-precision solver (nmx_ir_cfg.precision)
@@ -522,6 +522,83 @@ def solve_mpir(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int
         reports.append(info)
     x = xs[0] if B.dim() == 1 else torch.stack(xs, dim=1)
     return x, (reports[0] if B.dim() == 1 else reports)
+
+
+class MpirGraph:
+    """A captured :func:`solve_mpir` (one right-hand side, classical
+    refinement): :meth:`replay` solves with whatever ``a`` and ``b`` hold at
+    that moment into ``x`` with one graph launch and returns ``(x, report)``,
+    the report read back from the device (no timings)."""
+
+    def __init__(self, handle, ctx, a: torch.Tensor, b: torch.Tensor, x: torch.Tensor, eager_report: dict, keep):
+        self._handle = handle
+        self._ctx = ctx
+        self._keep = keep
+        self.a = a
+        self.b = b
+        self.x = x
+        self.eager_report = eager_report
+
+    def replay(self):
+        lib = _lib.load()
+        _lib.check(lib.nmx_graph_launch(self._handle), "graph replay")
+        rep = _lib.IrReport()
+        _lib.check(lib.nmx_ir_report_read(self._ctx.handle, ctypes.byref(rep)), "nmx_ir_report_read")
+        return self.x, _report(rep)
+
+    def close(self) -> None:
+        if self._handle:
+            _lib.load().nmx_graph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_mpir_graph(a, b, precision: str = "fp16s", tol: float = 1e-15, max_iter: int = 30, block: int = 256,
+                     pivot: str = "tournament") -> MpirGraph:
+    """:func:`solve_mpir` with classical refinement run once eagerly (which
+    sizes every buffer and creates the look-ahead stream), then captured into
+    a graph on the same stream; the eager solution is in ``graph.x`` and its
+    report in ``graph.eager_report`` until the first replay.  The graph holds
+    ``max_iter`` refinement steps, the ones after the device-side stop test
+    returning at once."""
+    A = _device_tensor(a, torch.float64).clone()
+    n = _square(A, "solve_mpir_graph")
+    Bv = _device_tensor(b, torch.float64).clone()
+    if Bv.dim() != 1 or Bv.shape[0] != n:
+        raise NumaxError("solve_mpir_graph: b must be a vector of the matrix's order")
+    if precision not in PRECISIONS:
+        raise NumaxError(f"solve_mpir_graph: precision must be one of {sorted(PRECISIONS)}")
+    if pivot not in PIVOTS:
+        raise NumaxError(f"solve_mpir_graph: pivot must be one of {sorted(PIVOTS)}")
+    lib = _lib.load()
+    cfg = _lib.IrCfg()
+    lib.nmx_ir_cfg_default(ctypes.byref(cfg))
+    cfg.precision = PRECISIONS[precision]
+    cfg.tol = tol
+    cfg.max_iter = max_iter
+    cfg.block = int(block)
+    cfg.pivot = PIVOTS[pivot]
+    cfg.solver = SOLVERS["classical"]
+    ctx = _ctx(A)
+    x = torch.empty_like(Bv)
+    rep = _lib.IrReport()
+    st = lib.nmx_gesv_mpir(ctx.handle, n, _ptr(A), n, _ptr(Bv), _ptr(x), ctypes.byref(cfg), ctypes.byref(rep))
+    if st not in (0, 5):
+        _lib.check(st, "solve_mpir_graph (eager)")
+    eager = _report(rep)
+    torch.cuda.synchronize(A.device)
+    _lib.check(lib.nmx_graph_begin(ctx.handle), "graph begin")
+    st = lib.nmx_gesv_mpir(ctx.handle, n, _ptr(A), n, _ptr(Bv), _ptr(x), ctypes.byref(cfg), None)
+    handle = ctypes.c_void_p()
+    st_end = lib.nmx_graph_end(ctx.handle, ctypes.byref(handle))
+    _lib.check(st, "solve_mpir (captured)")
+    _lib.check(st_end, "graph end")
+    return MpirGraph(handle, ctx, A, Bv, x, eager, cfg)
 
 
 # ---- batched uneven-size solves ---------------------------------------------

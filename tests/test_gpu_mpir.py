@@ -177,3 +177,65 @@ def test_autotuners_pick_from_their_candidates(tmp_path, monkeypatch):
     b = R.normal(2, (2048,))
     x, rep = nl.solve_mpir(a, b)
     assert rep["converged"] and oracle.backward_error(a, x.cpu().numpy(), b) <= 1e-15
+
+
+@pytest.mark.parametrize("pivot,kappa", [("tournament", 1e4), ("butterfly", 1e2)])
+def test_mpir_graph_replay_matches_eager_bit_for_bit(pivot, kappa):
+    """nmx_gesv_mpir with classical refinement captured whole (device-side
+    stop test, max_iter + 1 steps in the graph): every replay gives the eager
+    solve's x bit for bit and its report, with other eager solves on the same
+    context between replays, and a new b written into the captured buffer is
+    solved as an eager call solves it."""
+    import torch
+    n = 1024
+    a = svd_matrix(n, kappa, seed=21)
+    b = R.normal(22, (n,))
+    g = nl.solve_mpir_graph(a, b, max_iter=30, block=256, pivot=pivot)
+    x_e = g.x.clone()
+    rep_e = g.eager_report
+    assert rep_e["converged"] and 0 < rep_e["iterations"] < 30
+    x_ref, rep_ref = nl.solve_mpir(a, b, solver="classical", block=256, pivot=pivot)
+    assert torch.equal(x_ref, x_e) and rep_ref["iterations"] == rep_e["iterations"]
+    for k in range(3):
+        g.x.zero_()
+        x, rep = g.replay()
+        assert torch.equal(x, x_e), k
+        assert rep["converged"] and rep["iterations"] == rep_e["iterations"], (k, rep, rep_e)
+        assert rep["backward_error"] == rep_e["backward_error"]
+        assert rep["history"] == rep_e["history"]
+        nl.solve_mpir(a, 2.0 * b, solver="classical", block=256, pivot=pivot)  # eager work between replays
+    b2 = R.normal(23, (n,))
+    g.b.copy_(torch.as_tensor(b2, device=g.b.device))
+    x, rep = g.replay()
+    x2, rep2 = nl.solve_mpir(a, b2, solver="classical", block=256, pivot=pivot)
+    assert torch.equal(x, x2) and rep["iterations"] == rep2["iterations"]
+    assert oracle.backward_error(a, x.cpu().numpy(), b2) <= 1e-15
+    g.close()
+
+
+@pytest.mark.parametrize("solver,pivot", [(2, 2), (1, 1)])
+def test_mpir_capture_refuses_gmres_and_partial_pivoting(solver, pivot):
+    """GMRES-IR decides on the host and the partial-pivoting panel takes
+    host-side epochs: neither is captured."""
+    import ctypes
+
+    import torch
+    from numax_amd import _lib
+    n = 256
+    A = torch.as_tensor(svd_matrix(n, 1e2, seed=24), device="cuda")
+    b = torch.as_tensor(R.normal(25, (n,)), device="cuda")
+    x = torch.empty_like(b)
+    lib = _lib.load()
+    cfg = _lib.IrCfg()
+    lib.nmx_ir_cfg_default(ctypes.byref(cfg))
+    cfg.solver = solver
+    cfg.pivot = pivot
+    ctx = nl._ctx(A)
+    nl.solve_mpir(A, b, pivot="partial" if pivot == 1 else "tournament")  # buffers sized
+    torch.cuda.synchronize()
+    _lib.check(lib.nmx_graph_begin(ctx.handle), "begin")
+    st = lib.nmx_gesv_mpir(ctx.handle, n, nl._ptr(A), n, nl._ptr(b), nl._ptr(x), ctypes.byref(cfg), None)
+    h = ctypes.c_void_p()
+    _lib.check(lib.nmx_graph_end(ctx.handle, ctypes.byref(h)), "end")
+    lib.nmx_graph_destroy(h)
+    assert st == _lib.NMX_ERR_UNSUPPORTED
