@@ -35,6 +35,12 @@ ROOF = {"gemm": ("mfma", "fp64_TFs"), "mp_gemm": ("hbm", None), "getf2_leaf": ("
         "mp_panel": ("hbm", None), "potrf_diag": ("hbm", None), "other": ("hbm", None)}
 
 
+# the library's profiler classes are timed per class; the kernels each class
+# holds on the paths the roofline objects name
+CLASS_KERNELS = {("mp-ir", "getf2_leaf"): "tslu_tournament_kernel (tournament pivot selection of a 32-column leaf)",
+                 ("plain-fp64", "gemm"): "gemm_kernel<double> (trailing and in-panel updates)"}
+
+
 # HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, both factors from a
 # 1 GiB copy kernel) of the kernel classes a roofline object names, measured by
 # rocprofv3 --pmc on numax_amd/lib/nmx_profile at this benchmark's sizes:
@@ -584,7 +590,8 @@ def main() -> int:
             ach = rec["bytes"] / sec / 1e9
             peak = PEAK["hbm_GBs"]
             unit = "GB/s"
-        return {"kernel": name, "path": label, "bound": bound[0], "achieved": ach, "peak": peak, "unit": unit,
+        return {"kernel": name, "kernels": CLASS_KERNELS.get((label, name), name), "path": label, "bound": bound[0],
+                "achieved": ach, "peak": peak, "unit": unit,
                 "frac": ach / peak, "traffic": MEASURED_TRAFFIC.get((name, n)), "launches_per_step": rec["launches"],
                 "avg_launch_ms": rec["ms"] / max(rec["launches"], 1), "share_of_step": None}
 
