@@ -494,6 +494,95 @@ __global__ void laswp_kernel(T* __restrict__ A, int lda, int col_lo, int ncols, 
 // step costs a few dozen cycles instead of the two dependent global
 // round trips of a swap in place.  Pivot values are clamped to [0, n), so a
 // corrupt ipiv yields a wrong permutation but no out-of-range access.
+// The same permutation with the swaps taken 64 at a time.  A chunk's 64
+// swaps, applied in order, permute at most 128 positions: its own 64 and up
+// to 64 targets below it (ipiv[i] >= i).  A wavefront replays the chunk on
+// labels -- slot s holds position pos_s and, after each swap, which
+// position's value sits there (cont_s) -- so the chunk's net effect is
+// perm'[pos_s] = perm[cont_s] for every touched slot: all reads, then all
+// writes.  Sixteen wavefronts work out sixteen chunks at a time into LDS;
+// wavefront 0 then applies them in order to the permutation held in LDS.
+constexpr int kPermBatch = 16;  // chunks per round (one per wavefront)
+__global__ void __launch_bounds__(1024) perm_chunks_kernel(int n, const int32_t* __restrict__ ipiv,
+                                                           int32_t* __restrict__ perm) {
+  extern __shared__ int s[];  // the permutation, n entries
+  __shared__ int mdst[kPermBatch * 128], msrc[kPermBatch * 128], mcnt[kPermBatch];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = t; i < n; i += blockDim.x) s[i] = i;
+  const int nchunks = (n + 63) / 64;
+  for (int c0 = 0; c0 < nchunks; c0 += kPermBatch) {
+    const int c = c0 + wv;
+    if (wv < kPermBatch && c < nchunks) {
+      const int p0 = 64 * c, m = min(64, n - p0);
+      const int iv = lane < m ? min(max(ipiv[p0 + lane], 0), n - 1) : p0 + lane;
+      int pos0 = p0 + lane, cont0 = pos0;  // slot lane: position p0 + lane
+      int pos1 = -1, cont1 = -1;           // slot 64 + lane: a target outside the chunk
+      int nhi = 0;
+      for (int j = 0; j < m; j++) {
+        const int q = __builtin_amdgcn_readlane(iv, j);
+        if (q == p0 + j) continue;
+        int tq;
+        if (q >= p0 && q < p0 + 64) {
+          tq = q - p0;
+        } else {
+          const unsigned long long hit = __ballot(lane < nhi && pos1 == q);
+          if (hit) {
+            tq = 64 + __ffsll((long long)hit) - 1;
+          } else {
+            tq = 64 + nhi;
+            if (lane == nhi) {
+              pos1 = q;
+              cont1 = q;
+            }
+            nhi++;
+          }
+        }
+        const int a = __builtin_amdgcn_readlane(cont0, j);
+        const int b = tq < 64 ? __builtin_amdgcn_readlane(cont0, tq) : __builtin_amdgcn_readlane(cont1, tq - 64);
+        if (lane == j) cont0 = b;
+        if (tq < 64) {
+          if (lane == tq) cont0 = a;
+        } else if (lane == tq - 64) {
+          cont1 = a;
+        }
+      }
+      // compact the moved slots
+      const bool mv0 = cont0 != pos0, mv1 = lane < nhi && cont1 != pos1;
+      const unsigned long long b0 = __ballot(mv0), b1 = __ballot(mv1);
+      const int base1 = __popcll(b0);
+      int* d = mdst + wv * 128;
+      int* sr = msrc + wv * 128;
+      if (mv0) {
+        const int at = __popcll(b0 & ((1ull << lane) - 1ull));
+        d[at] = pos0;
+        sr[at] = cont0;
+      }
+      if (mv1) {
+        const int at = base1 + __popcll(b1 & ((1ull << lane) - 1ull));
+        d[at] = pos1;
+        sr[at] = cont1;
+      }
+      if (lane == 0) mcnt[wv] = base1 + __popcll(b1);
+    }
+    __syncthreads();  // this round's moves, and (first round) the identity, are in LDS
+    if (wv == 0) {
+      for (int cc = 0; cc < kPermBatch && c0 + cc < nchunks; cc++) {
+        const int cnt = mcnt[cc];
+        const int* d = mdst + cc * 128;
+        const int* sr = msrc + cc * 128;
+        const int v0 = lane < cnt ? s[sr[lane]] : 0;
+        const int v1 = lane + 64 < cnt ? s[sr[lane + 64]] : 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // every read before any write
+        if (lane < cnt) s[d[lane]] = v0;
+        if (lane + 64 < cnt) s[d[lane + 64]] = v1;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      }
+    }
+    __syncthreads();  // the moves' LDS is free for the next round
+  }
+  for (int i = t; i < n; i += blockDim.x) perm[i] = s[i];
+}
+
 template <int IN_LDS>
 __global__ void __launch_bounds__(1024) perm_kernel(int n, const int32_t* __restrict__ ipiv,
                                                      int32_t* __restrict__ perm) {
@@ -989,7 +1078,7 @@ int perm_from_ipiv(nmx_ctx* ctx, int n, const int32_t* ipiv, int32_t* perm) {
   ProfScope ps(ctx, K_LASWP, 0.0, 8.0 * n);
   const size_t lds = sizeof(int32_t) * (size_t)n;
   if (lds <= kPermLdsBytes)
-    perm_kernel<1><<<1, 1024, lds, ctx->stream>>>(n, ipiv, perm);
+    perm_chunks_kernel<<<1, 1024, lds, ctx->stream>>>(n, ipiv, perm);
   else
     perm_kernel<0><<<1, 1024, 0, ctx->stream>>>(n, ipiv, perm);
   NMX_HIP(ctx, hipGetLastError());
