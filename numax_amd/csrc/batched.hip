@@ -89,19 +89,41 @@ struct BatchArgs {
   double tol;
 };
 
+// A wavefront's LDS, sized for its class's largest order NMAX = 64 Q
+template <int NMAX>
 struct WaveLds {
-  float xs[kBatchMaxN];          // FP32 solve vector
-  double x64[kBatchMaxN];        // solution
-  double r64[kBatchMaxN];        // residual
-  int perm[kBatchMaxN];          // original row now at row i (LU)
-  int ipiv[kBatchMaxN];          // interchanges, LAPACK order
-  float rowbuf[2][kNB];          // interchange staging
+  float xs[NMAX];         // FP32 solve vector; the residual is written here in solve order
+  double x64[NMAX];       // solution
+  int perm[NMAX];         // original row now at row i (LU)
+  int ipiv[NMAX];         // interchanges, LAPACK order
+  float rowbuf[2][kNB];   // interchange staging
   int mv_dst[2 * kNB], mv_src[2 * kNB];
 };
-// the rolled solves (n <= 128) stage each 64 x 64 diagonal block in LDS
-struct WaveLdsBlk : WaveLds {
+// the rolled solves (n <= 128) stage each 64 x 64 diagonal block in LDS;
+// at 20 KB or less a wavefront, two workgroups of four fit a CU's 160 KB
+template <int NMAX>
+struct WaveLdsBlk : WaveLds<NMAX> {
   float blk[64][65];
 };
+static_assert(sizeof(WaveLdsBlk<128>) * kWaves <= 160 * 1024 / 2, "two workgroups a CU");
+// the unrolled classes (n > 128) need LDS only for the staging tile
+constexpr int kTileLd = 33;  // staging tile row stride (words): rows and columns both conflict-free
+template <int NMAX>
+struct WaveLdsTile : WaveLds<NMAX> {
+  float tt[32 * kTileLd];
+};
+// a 32 x 33 staging tile: its own array, or the start of the solve block
+template <int NMAX>
+__device__ __forceinline__ float* tile_of(WaveLdsBlk<NMAX>& S) { return &S.blk[0][0]; }
+template <int NMAX>
+__device__ __forceinline__ float* tile_of(WaveLdsTile<NMAX>& S) { return S.tt; }
+static_assert(sizeof(WaveLdsBlk<64>::blk) >= 32 * kTileLd * sizeof(float), "staging tile fits the solve block");
+
+// Orders a wavefront's accesses to its own staging tile: the LDS executes a
+// wavefront's instructions in order, so waiting for its LDS counter (and
+// keeping the compiler from moving accesses across) is enough; unlike
+// wave_sync it does not wait for global loads and stores in flight.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // Every row held in registers is rotated left by one column after each
 // step of a rolled column loop, so the current column is always register 0
@@ -121,8 +143,8 @@ __device__ __forceinline__ void rot1(float (&v)[N]) {
 // with the diagonal as the incumbent (a NaN incumbent is kept, a NaN never
 // displaces one), full-row interchange, pivot guard_nonzero(a_kk),
 // multipliers by true division.
-template <int Q>
-__device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+template <int Q, class L>
+__device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, int nb, L& S, int lane,
                                          int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float pv[Q][kNB];
@@ -225,8 +247,8 @@ __device__ __forceinline__ void panel_lu(float* __restrict__ F, int n, int k, in
 // The panel's interchanges ipiv[k..k+nb) over the columns outside it: the
 // net row moves (<= 2 nb rows) are worked out once by the wavefront, then
 // each lane moves whole columns (all loads before the stores).
-template <int Q>
-__device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane) {
+template <int Q, class L>
+__device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k, int nb, L& S, int lane) {
   int myrow = lane < nb ? k + lane : -1;
   int mine = myrow;
   int nr = nb;
@@ -282,7 +304,8 @@ __device__ __forceinline__ void swap_outside(float* __restrict__ F, int n, int k
 // L11 with the lane's columns (x) and L11's rows (lane i < nb holds row i)
 // both rotated, so L[jj + c][jj] is lane jj + c's register 0.
 template <int Q>
-__device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, int nb, int lane) {
+__device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, int nb, float* __restrict__ tt,
+                                         int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float l11[kNB];
   {
@@ -332,8 +355,8 @@ __device__ __forceinline__ void trsm_u12(float* __restrict__ F, int n, int k, in
 // _array/cholesky.mojo:26-73's arithmetic per column: the diagonal
 // guard_nonzero'ed before the square root (a negative pivot gives NaN), the
 // column divided by ljj; right-looking here, rows rotated as in the LU panel.
-template <int Q>
-__device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+template <int Q, class L>
+__device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, int nb, L& S, int lane,
                                            int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float dv[kNB];
@@ -419,8 +442,8 @@ __device__ __forceinline__ void panel_chol(float* __restrict__ F, int n, int k, 
 // instance, so every register index is static however large the unrolled
 // panel gets (a partially unrolled loop would index pv dynamically and move
 // it to scratch memory).
-template <int Q, int J>
-__device__ __forceinline__ void lu_column(float (&pv)[Q][kNB], int n, int k, WaveLds& S, int lane, int& info) {
+template <int Q, int J, class L>
+__device__ __forceinline__ void lu_column(float (&pv)[Q][kNB], int n, int k, L& S, int lane, int& info) {
   const int kj = k + J;
   uint32_t key = 0;
   int crow = INT_MAX;
@@ -490,8 +513,8 @@ __device__ __forceinline__ void lu_column(float (&pv)[Q][kNB], int n, int k, Wav
   }
 }
 
-template <int Q, int J>
-__device__ __forceinline__ void lu_columns(float (&pv)[Q][kNB], int n, int k, int nb, WaveLds& S, int lane,
+template <int Q, int J, class L>
+__device__ __forceinline__ void lu_columns(float (&pv)[Q][kNB], int n, int k, int nb, L& S, int lane,
                                            int& info) {
   if constexpr (J < kNB) {
     if (J < nb) {
@@ -501,8 +524,8 @@ __device__ __forceinline__ void lu_columns(float (&pv)[Q][kNB], int n, int k, in
   }
 }
 
-template <int Q>
-__device__ __forceinline__ void panel_lu_unrolled(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+template <int Q, class L>
+__device__ __forceinline__ void panel_lu_unrolled(float* __restrict__ F, int n, int k, int nb, L& S, int lane,
                                          int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float pv[Q][kNB];
@@ -530,7 +553,8 @@ __device__ __forceinline__ void panel_lu_unrolled(float* __restrict__ F, int n, 
 // copy read by all lanes gets all 496 reads hoisted ahead of the arithmetic
 // and spills), and is applied to the lane's Q columns.
 template <int Q>
-__device__ __forceinline__ void trsm_u12_unrolled(float* __restrict__ F, int n, int k, int nb, int lane) {
+__device__ __forceinline__ void trsm_u12_unrolled(float* __restrict__ F, int n, int k, int nb,
+                                                  float* __restrict__ tt, int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float l11[kNB];
   {
@@ -571,15 +595,14 @@ __device__ __forceinline__ void trsm_u12_unrolled(float* __restrict__ F, int n, 
 // _array/cholesky.mojo:26-73's arithmetic per column: the diagonal
 // guard_nonzero'ed before the square root (a negative pivot gives NaN), the
 // column divided by ljj; right-looking here.
-template <int Q>
-__device__ __forceinline__ void panel_chol_unrolled(float* __restrict__ F, int n, int k, int nb, WaveLds& S, int lane,
+template <int Q, class L>
+__device__ __forceinline__ void panel_chol_unrolled(float* __restrict__ F, int n, int k, int nb, L& S, int lane,
                                            int& info) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   float dv[kNB];
   {
-    const int row = k + lane;
     const bool own = lane < nb;
-    const uint32_t ro = (uint32_t)(own ? row : 0) * (uint32_t)n + (uint32_t)k;
+    const uint32_t ro = (uint32_t)(k + (own ? lane : 0)) * (uint32_t)n + (uint32_t)k;
 #pragma unroll
     for (int c = 0; c < kNB; c++) dv[c] = buf_load<float>(rf, (own & (c < nb)) ? 4u * (ro + c) : kBufOob);
   }
@@ -641,8 +664,35 @@ __device__ __forceinline__ void panel_chol_unrolled(float* __restrict__ F, int n
 // C -= L21 U12 (LU) or the lower tiles of C -= L21 L21^T (Cholesky), 32 x 32
 // tiles of v_mfma_f32_32x32x2f32 (A element: lane l holds A[l%32][l/32]; B:
 // B[l/32][l%32]; D item i: row 8 (i/4) + 4 (l/32) + i%4, column l%32).
+// The A operand (a 32-row tile of L21, and the B operand of a Cholesky
+// update, which is the same layout of another L21 tile) wants a row per lane
+// pair: it is loaded as 128-byte row segments and rearranged in the staging
+// tile.
+__device__ __forceinline__ void l21_raw(__amdgpu_buffer_rsrc_t rf, int n, int k, int nb, int row0, int lane,
+                                        float (&raw)[16]) {
+  const int cl = lane & 31, rh = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = row0 + 2 * i + rh;
+    raw[i] = buf_load<float>(rf, ((r < n) & (cl < nb)) ? 4u * ((uint32_t)r * (uint32_t)n + (uint32_t)(k + cl))
+                                                       : kBufOob);
+  }
+}
+// raw (row 2 i + l / 32, column l % 32 of the tile) to the MFMA layout
+// (row l % 32, column 2 s + l / 32)
+__device__ __forceinline__ void l21_mfma(float* __restrict__ tt, int lane, const float (&raw)[16], float (&a)[16]) {
+  const int cl = lane & 31, rh = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) tt[(2 * i + rh) * kTileLd + cl] = raw[i];
+  lds_sync();
+#pragma unroll
+  for (int s = 0; s < 16; s++) a[s] = tt[cl * kTileLd + 2 * s + rh];
+  lds_sync();
+}
+
 template <bool CHOL>
-__device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, int k, int nb, int lane) {
+__device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, int k, int nb, float* __restrict__ tt,
+                                                int lane) {
   const int r0 = k + nb;
   const int m2 = n - r0;
   if (m2 <= 0) return;
@@ -653,24 +703,18 @@ __device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, in
   for (int tj = 0; tj < tiles; tj++) {
     const int col = r0 + 32 * tj + l32;
     float b[16];
-#pragma unroll
-    for (int s = 0; s < 16; s++) {
-      const int kk = 2 * s + kh;
-      const uint32_t off = CHOL ? (uint32_t)col * (uint32_t)n + (uint32_t)(k + kk)
-                                : (uint32_t)(k + kk) * (uint32_t)n + (uint32_t)col;
-      b[s] = buf_load<float>(rf, ((kk < nb) & (col < n)) ? 4u * off : kBufOob);
-    }
-    // tile ti's operands are loaded while tile ti-1 is on the matrix cores
-    auto load_a = [&](int ti, float (&a)[16]) {
-      const int arow = r0 + 32 * ti + l32;
+    if (CHOL) {
+      float raw[16];
+      l21_raw(rf, n, k, nb, r0 + 32 * tj, lane, raw);
+      l21_mfma(tt, lane, raw, b);
+    } else {
 #pragma unroll
       for (int s = 0; s < 16; s++) {
         const int kk = 2 * s + kh;
-        a[s] = buf_load<float>(rf, ((kk < nb) & (arow < n) & (ti < tiles))
-                                       ? 4u * ((uint32_t)arow * (uint32_t)n + (uint32_t)(k + kk))
-                                       : kBufOob);
+        b[s] = buf_load<float>(rf, ((kk < nb) & (col < n)) ? 4u * ((uint32_t)(k + kk) * (uint32_t)n + (uint32_t)col)
+                                                           : kBufOob);
       }
-    };
+    }
     auto load_c = [&](int ti, floatx16& c) {
 #pragma unroll
       for (int i = 0; i < 16; i++) {
@@ -681,13 +725,19 @@ __device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, in
       }
     };
     const int t0 = CHOL ? tj : 0;
-    float a[16], an[16];
+    float a[16];
     floatx16 acc, cn;
-    load_a(t0, a);
+    {
+      float raw[16];
+      l21_raw(rf, n, k, nb, r0 + 32 * t0, lane, raw);
+      l21_mfma(tt, lane, raw, a);
+    }
     load_c(t0, acc);
+    // tile ti + 1's operands are loaded while tile ti is on the matrix cores
 #pragma unroll 1
     for (int ti = t0; ti < tiles; ti++) {
-      load_a(ti + 1, an);
+      float rawn[16];
+      l21_raw(rf, n, k, nb, r0 + 32 * (ti + 1), lane, rawn);
       load_c(ti + 1, cn);
 #pragma unroll
       for (int s = 0; s < 16; s++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(-a[s], b[s], acc, 0, 0, 0);
@@ -696,8 +746,7 @@ __device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, in
         const int row = r0 + 32 * ti + 8 * (i >> 2) + 4 * kh + (i & 3);
         if (row < n && col < n) F[(int64_t)row * n + col] = acc[i];
       }
-#pragma unroll
-      for (int s = 0; s < 16; s++) a[s] = an[s];
+      l21_mfma(tt, lane, rawn, a);
       acc = cn;
     }
   }
@@ -708,8 +757,8 @@ __device__ __forceinline__ void trailing_update(float* __restrict__ F, int n, in
 // in LDS and swept by a rolled loop (lane c's value reaches the others as a
 // scalar read), then the other rows are updated by one dot product per lane
 // and slot.  TRANS reads F transposed (L^T).
-template <int Q, bool LOWER, bool UNIT, bool TRANS>
-__device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, WaveLdsBlk& S, int lane) {
+template <int Q, bool LOWER, bool UNIT, bool TRANS, class L>
+__device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, L& S, int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   const int nbk = (n + 63) >> 6;
 #pragma unroll 1
@@ -786,8 +835,8 @@ __device__ __forceinline__ void tri_solve(const float* __restrict__ F, int n, Wa
 // op(F) x = xs in place, 64 rows at a time: the diagonal block with every
 // index static (values exchanged through scalar reads), then the other rows
 // by one dot product per lane and slot.  TRANS reads F transposed (L^T).
-template <int Q, bool LOWER, bool UNIT, bool TRANS>
-__device__ __forceinline__ void tri_solve_unrolled(const float* __restrict__ F, int n, WaveLds& S, int lane) {
+template <int Q, bool LOWER, bool UNIT, bool TRANS, class L>
+__device__ __forceinline__ void tri_solve_unrolled(const float* __restrict__ F, int n, L& S, int lane) {
   const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
   const int nbk = (n + 63) >> 6;
 #pragma unroll 1
@@ -868,6 +917,80 @@ __device__ __forceinline__ void tri_solve_unrolled(const float* __restrict__ F, 
   }
 }
 
+// After the factorization, F is rewritten for the solves, which read it
+// only as op(F) = F^T (a load instruction then covers consecutive words
+// instead of one row per lane): an LU factor L\U is transposed in place; a
+// Cholesky factor, whose strict upper triangle is unused, gets L^T there
+// (F = L + strict_upper(L^T), so op(F)'s lower triangle is L and its upper
+// L^T).  32 x 32 tiles go through a 32 x 33 LDS tile: lane l covers row
+// 2 i + l / 32 and column l % 32 of a tile, i = 0..15, for both the load and
+// the store, so every global access is two 128-byte rows.
+__device__ __forceinline__ void tile_load(__amdgpu_buffer_rsrc_t rf, int n, int I, int J, int lane, float (&v)[16]) {
+  const int cl = lane & 31, rh = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = 32 * I + 2 * i + rh, c = 32 * J + cl;
+    v[i] = buf_load<float>(rf, ((r < n) & (c < n)) ? 4u * ((uint32_t)r * (uint32_t)n + (uint32_t)c) : kBufOob);
+  }
+}
+// v becomes its transpose (v[i] at row 2 i + l / 32, column l % 32 then
+// holds the entry at row l % 32, column 2 i + l / 32)
+__device__ __forceinline__ void tile_xpose(float* __restrict__ tt, int lane, float (&v)[16]) {
+  const int cl = lane & 31, rh = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) tt[(2 * i + rh) * 33 + cl] = v[i];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = tt[cl * 33 + 2 * i + rh];
+  wave_sync();
+}
+__device__ __forceinline__ void tile_store(__amdgpu_buffer_rsrc_t rf, int n, int I, int J, int lane,
+                                           const float (&v)[16], bool upper_only) {
+  const int cl = lane & 31, rh = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = 32 * I + 2 * i + rh, c = 32 * J + cl;
+    const bool ok = (r < n) & (c < n) & (!upper_only | (c > r));
+    buf_store(rf, ok ? 4u * ((uint32_t)r * (uint32_t)n + (uint32_t)c) : kBufOob, v[i]);
+  }
+}
+
+template <bool CHOL>
+__device__ __forceinline__ void factor_for_solves(float* __restrict__ F, int n, float* __restrict__ tt, int lane) {
+  const __amdgpu_buffer_rsrc_t rf = buf_rsrc(F);
+  const int nt = (n + 31) >> 5;
+#pragma unroll 1
+  for (int I = 0; I < nt; I++) {
+    if (CHOL) {
+#pragma unroll 1
+      for (int J = 0; J <= I; J++) {  // tile (I, J) of L, transposed, into (J, I); above the diagonal only
+        float v[16];
+        tile_load(rf, n, I, J, lane, v);
+        tile_xpose(tt, lane, v);
+        tile_store(rf, n, J, I, lane, v, I == J);
+      }
+    } else {
+      {
+        float v[16];
+        tile_load(rf, n, I, I, lane, v);
+        tile_xpose(tt, lane, v);
+        tile_store(rf, n, I, I, lane, v, false);
+      }
+#pragma unroll 1
+      for (int J = I + 1; J < nt; J++) {  // swap (I, J) and (J, I), each transposed
+        float v[16], w[16];
+        tile_load(rf, n, I, J, lane, v);
+        tile_load(rf, n, J, I, lane, w);
+        tile_xpose(tt, lane, v);
+        tile_xpose(tt, lane, w);
+        tile_store(rf, n, J, I, lane, v, false);
+        tile_store(rf, n, I, J, lane, w, false);
+      }
+    }
+  }
+  wave_sync();
+}
+
 // The classes with n <= 128 run the rolled phases (compact code, so the
 // instruction cache holds a wavefront's loop); the classes with n > 128 run
 // the fully unrolled ones.
@@ -876,60 +999,147 @@ constexpr bool kRolled = Q <= 2;
 
 template <int Q, bool CHOL, class L>
 __device__ __forceinline__ void solve_wave(const float* __restrict__ F, int n, L& S, int lane) {
+  // F holds the factor transposed (LU) or mirrored (Cholesky), so both
+  // triangles are read as op(F) = F^T: consecutive lanes, consecutive words
   if constexpr (kRolled<Q>) {
-    if (CHOL) {
-      tri_solve<Q, true, false, false>(F, n, S, lane);
-      tri_solve<Q, false, false, true>(F, n, S, lane);
-    } else {
-      tri_solve<Q, true, true, false>(F, n, S, lane);
-      tri_solve<Q, false, false, false>(F, n, S, lane);
-    }
+    tri_solve<Q, true, !CHOL, true>(F, n, S, lane);
+    tri_solve<Q, false, false, true>(F, n, S, lane);
   } else {
-    if (CHOL) {
-      tri_solve_unrolled<Q, true, false, false>(F, n, S, lane);
-      tri_solve_unrolled<Q, false, false, true>(F, n, S, lane);
-    } else {
-      tri_solve_unrolled<Q, true, true, false>(F, n, S, lane);
-      tri_solve_unrolled<Q, false, false, false>(F, n, S, lane);
-    }
+    tri_solve_unrolled<Q, true, !CHOL, true>(F, n, S, lane);
+    tri_solve_unrolled<Q, false, false, true>(F, n, S, lane);
   }
 }
 
-// r64 = b - A x64 in FP64 (a Cholesky problem is the symmetric matrix its
-// lower triangle defines); returns ||r||_inf / (||A||_inf ||x||_inf).
-template <int Q, bool CHOL>
-__device__ __forceinline__ double residual_wave(const double* __restrict__ A, const double* __restrict__ b, int n,
-                                                bool first, double& anorm, WaveLds& S, int lane) {
-  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A);
-  double rmax = 0.0, xmax = 0.0, amax = 0.0;
-#pragma unroll 1
-  for (int s = 0; s < Q; s++) {
-    const int row = lane + 64 * s;
-    if (row < n) {
-      double part = 0.0, asum = 0.0;
-      const uint32_t ro = (uint32_t)row * (uint32_t)n;
-#pragma unroll 1
-      for (int j0 = 0; j0 < n; j0 += 16) {
-        double av[16];
+// Sums of v[i] over the 16 lanes of each row of 16 lanes, scattered: lane c
+// of a row returns the sum for i = c.  Four halving exchanges (lane c keeps
+// the half of its values whose index has c's bit and sends the other half to
+// lane c ^ bit), 15 shuffles in all instead of 16 reductions.
+__device__ __forceinline__ double reduce_scatter16(const double (&v)[16], int lane) {
+  const int c = lane & 15;
+  double w8[8], w4[4], w2[2];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const int j = j0 + u;
-          const uint32_t off = (CHOL && j > row) ? (uint32_t)j * (uint32_t)n + (uint32_t)row : ro + (uint32_t)j;
-          av[u] = buf_load<double>(ra, j < n ? 8u * off : kBufOob);
+  for (int t = 0; t < 8; t++) {
+    const bool hi = c & 8;
+    w8[t] = (hi ? v[t + 8] : v[t]) + __shfl_xor(hi ? v[t] : v[t + 8], 8);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const bool hi = c & 4;
+    w4[t] = (hi ? w8[t + 4] : w8[t]) + __shfl_xor(hi ? w8[t] : w8[t + 4], 4);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const bool hi = c & 2;
+    w2[t] = (hi ? w4[t + 2] : w4[t]) + __shfl_xor(hi ? w4[t] : w4[t + 2], 2);
+  }
+  const bool hi = c & 1;
+  return (hi ? w2[1] : w2[0]) + __shfl_xor(hi ? w2[0] : w2[1], 1);
+}
+
+// r = b - A x64 in FP64 (a Cholesky problem is the symmetric matrix its
+// lower triangle defines), rounded to FP32 into xs in the order the next
+// solve takes its right-hand side (row perm[p] at p); returns
+// ||r||_inf / (||A||_inf ||x||_inf).  ||A||_inf is summed on the first call.
+// Positions go 64 at a time; lane l covers positions g + 4 i + l / 16
+// (i = 0..15) and columns c0 + l % 16 of each 16-column chunk, so a load
+// instruction reads four 128-byte row segments, and the 16 partial sums per
+// lane are combined once per group by reduce_scatter16.  The strict upper
+// triangle of a Cholesky matrix is read as columns of its lower triangle:
+// lane l then covers positions g + 16 i + l % 16 (i = 0..3) and rows
+// j0 + l / 16 of A, four 128-byte column segments a load.
+template <int Q, bool CHOL, bool FIRST, class L>
+__device__ __forceinline__ double residual_wave(const double* __restrict__ A, const double* __restrict__ b, int n,
+                                                double& anorm, L& S, int lane) {
+  constexpr bool first = FIRST;
+  const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A);
+  const int cl = lane & 15, rq = lane >> 4;
+  double rmax = 0.0, amax = 0.0;
+#pragma unroll 1
+  for (int g = 0; g < n; g += 64) {
+    int rowi[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int p = g + 4 * i + rq;
+      rowi[i] = p < n ? (CHOL ? p : S.perm[p]) : -1;
+    }
+    double acc[16], as[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = as[i] = 0.0;
+    const int cend = CHOL ? min(n, g + 64) : n;  // Cholesky: columns j <= row here
+#pragma unroll 1
+    for (int c0 = 0; c0 < cend; c0 += 16) {
+      const int j = c0 + cl;
+      const double xj = S.x64[j < n ? j : 0];
+      double av[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = rowi[i];
+        const bool ok = (row >= 0) & (j < n) & (!CHOL | (j <= row));
+        av[i] = buf_load<double>(ra, ok ? 8u * ((uint32_t)row * (uint32_t)n + (uint32_t)j) : kBufOob);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        acc[i] += av[i] * xj;
+        if (first) as[i] += fabs(av[i]);
+      }
+    }
+    double sum = reduce_scatter16(acc, lane);
+    double asum = 0.0;
+    if constexpr (first) asum = reduce_scatter16(as, lane);
+    const int p = g + 4 * cl + rq;  // this lane's position after the scatter
+    if (CHOL) {
+      const int pr = lane & 15, jq = lane >> 4;
+      double up[4] = {0.0, 0.0, 0.0, 0.0}, ua[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int j0 = g; j0 < n; j0 += 4) {
+        const int jj = j0 + jq;
+        const double xj = S.x64[jj < n ? jj : 0];
+        double av[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int pp = g + 16 * i + pr;
+          const bool ok = (pp < n) & (jj < n) & (jj > pp);
+          av[i] = buf_load<double>(ra, ok ? 8u * ((uint32_t)jj * (uint32_t)n + (uint32_t)pp) : kBufOob);
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const int j = j0 + u;
-          part += av[u] * S.x64[j < n ? j : 0];
-          asum += fabs(av[u]);
+        for (int i = 0; i < 4; i++) {
+          up[i] += av[i] * xj;
+          if (first) ua[i] += fabs(av[i]);
         }
       }
-      const double r = b[row] - part;
-      S.r64[row] = r;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        up[i] += __shfl_xor(up[i], 16);
+        up[i] += __shfl_xor(up[i], 32);
+        if (first) {
+          ua[i] += __shfl_xor(ua[i], 16);
+          ua[i] += __shfl_xor(ua[i], 32);
+        }
+      }
+      // position p = g + 16 (m / 16) + m % 16 with m = 4 cl + rq: entry m / 16 of lane m % 16
+      const int m = 4 * cl + rq, src = m & 15, e = m >> 4;
+      double u = 0.0, a = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const double ui = __shfl(up[i], src);
+        const double ai = first ? __shfl(ua[i], src) : 0.0;
+        if (e == i) { u = ui; a = ai; }
+      }
+      sum += u;
+      asum += a;
+    }
+    if (p < n) {
+      const double r = b[CHOL ? p : S.perm[p]] - sum;
+      S.xs[p] = (float)r;
       rmax = fmax(rmax, fabs(r));
-      xmax = fmax(xmax, fabs(S.x64[row]));
       amax = fmax(amax, asum);
     }
+  }
+  double xmax = 0.0;
+#pragma unroll 1
+  for (int s = 0; s < Q; s++) {
+    const int i = lane + 64 * s;
+    if (i < n) xmax = fmax(xmax, fabs(S.x64[i]));
   }
   rmax = wave_dmax(rmax);
   xmax = wave_dmax(xmax);
@@ -939,9 +1149,14 @@ __device__ __forceinline__ double residual_wave(const double* __restrict__ A, co
   return den > 0.0 ? rmax / den : (rmax == 0.0 ? 0.0 : INFINITY);
 }
 
+// Two wavefronts per SIMD: left to itself the compiler takes 256 VGPRs and
+// 16-28 AGPRs, which leaves room for one, and a lone wavefront waits out
+// every load of its chain of dependent phases.  Held to 256 registers the
+// larger classes spill a few words a lane.
 template <int Q, bool CHOL>
-__global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs args, int cls) {
-  using Lds = typename std::conditional<kRolled<Q>, WaveLdsBlk, WaveLds>::type;
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2)))
+batched_wave_kernel(BatchArgs args, int cls) {
+  using Lds = typename std::conditional<kRolled<Q>, WaveLdsBlk<64 * Q>, WaveLdsTile<64 * Q>>::type;
   __shared__ Lds lds[kWaves];
   const int lane = threadIdx.x & 63;
   // the wavefront index as a scalar, so every LDS address below is one base
@@ -1007,26 +1222,26 @@ __global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs arg
         wave_sync();
         swap_outside<Q>(F, n, k, nb, S, lane);
         if (k + nb < n) {
-          if constexpr (kRolled<Q>) trsm_u12<Q>(F, n, k, nb, lane);
-          else trsm_u12_unrolled<Q>(F, n, k, nb, lane);
+          if constexpr (kRolled<Q>) trsm_u12<Q>(F, n, k, nb, tile_of(S), lane);
+          else trsm_u12_unrolled<Q>(F, n, k, nb, tile_of(S), lane);
         }
       }
-      trailing_update<CHOL>(F, n, k, nb, lane);
+      trailing_update<CHOL>(F, n, k, nb, tile_of(S), lane);
     }
+    factor_for_solves<CHOL>(F, n, tile_of(S), lane);
 
     // x0 = op^-1 P b in FP32, then FP64 refinement (one solve site)
     double anorm = 0.0, eta = 0.0, prev = INFINITY;
     int it = 0;
     for (bool first = true;; first = false) {
+      if (first) {  // later right-hand sides are the residuals residual_wave left in xs
 #pragma unroll 1
-      for (int s = 0; s < Q; s++) {
-        const int row = lane + 64 * s;
-        if (row < n) {
-          const int src = CHOL ? row : S.perm[row];
-          S.xs[row] = first ? (float)b[src] : (float)S.r64[src];
+        for (int s = 0; s < Q; s++) {
+          const int row = lane + 64 * s;
+          if (row < n) S.xs[row] = (float)b[CHOL ? row : S.perm[row]];
         }
+        wave_sync();
       }
-      wave_sync();
       solve_wave<Q, CHOL>(F, n, S, lane);
 #pragma unroll 1
       for (int s = 0; s < Q; s++) {
@@ -1034,7 +1249,8 @@ __global__ void __launch_bounds__(64 * kWaves) batched_wave_kernel(BatchArgs arg
         if (row < n) S.x64[row] = first ? (double)S.xs[row] : S.x64[row] + (double)S.xs[row];
       }
       wave_sync();
-      eta = residual_wave<Q, CHOL>(A, b, n, first, anorm, S, lane);
+      eta = first ? residual_wave<Q, CHOL, true>(A, b, n, anorm, S, lane)
+                  : residual_wave<Q, CHOL, false>(A, b, n, anorm, S, lane);
       if (!first) it++;
       if (!(eta > args.tol) || it >= args.max_iter || !(eta < 0.5 * prev)) break;
       prev = eta;
@@ -1214,10 +1430,12 @@ static BatchWs batched_ws(int count, int64_t a_elems) {
   return w;
 }
 
-// The solve itself, in the scratch `ws` (batched_ws bytes) the caller found.
+// The solve itself, in the scratch `ws` (batched_ws bytes) the caller found;
+// `two_streams` lets the Cholesky classes run on the context's second stream.
 static int batched_run(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA, const int64_t* offB,
                        const uint8_t* kind, const double* A, int64_t a_elems, const double* B, int64_t b_elems,
-                       double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg& cfg, void* ws) {
+                       double* X, int32_t* info, int32_t* iters, double* berr, const nmx_batch_cfg& cfg, void* ws,
+                       bool two_streams) {
   const BatchWs wl = batched_ws(count, a_elems);
   float* F = (float*)ws;
   int64_t* offF = (int64_t*)((char*)ws + wl.f_bytes);
@@ -1240,15 +1458,40 @@ static int batched_run(nmx_ctx* ctx, int count, const int32_t* n, const int64_t*
   BatchArgs a{count, n, offA, offB, kind, A, B, X, info, iters, berr, F, offF, order, cls, head, cfg.max_iter, cfg.tol};
   {
     ProfScope ps(ctx, K_BATCH, 0.0, 8.0 * (double)a_elems);
-    // largest problems first; LU then Cholesky within a size class
+    // Largest problems first.  Each class is one persistent launch, and
+    // launches on one stream run one after another, so a class's last
+    // problems would leave most of the chip idle; with two streams the
+    // Cholesky classes fill the slots the LU classes free (and the reverse).
+    // Problems are independent, so the results do not depend on the split.
+    const bool split = two_streams && (ctx->aux || !ctx->capturing) && nmx_aux_stream(ctx) == NMX_OK;
+    if (split) {
+      NMX_HIP(ctx, hipEventRecord(ctx->ahead_ev, ctx->stream));
+      NMX_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->ahead_ev, 0));
+    }
     NMX_TRY((launch_class<4, false>(ctx, a, cfg.teams_per_cu[0])));
-    NMX_TRY((launch_class<4, true>(ctx, a, cfg.teams_per_cu[0])));
+    {
+      StreamSwap on(ctx, split ? ctx->aux : ctx->stream);
+      NMX_TRY((launch_class<4, true>(ctx, a, cfg.teams_per_cu[0])));
+    }
     NMX_TRY((launch_class<3, false>(ctx, a, cfg.teams_per_cu[0])));
-    NMX_TRY((launch_class<3, true>(ctx, a, cfg.teams_per_cu[0])));
+    {
+      StreamSwap on(ctx, split ? ctx->aux : ctx->stream);
+      NMX_TRY((launch_class<3, true>(ctx, a, cfg.teams_per_cu[0])));
+    }
     NMX_TRY((launch_class<2, false>(ctx, a, cfg.teams_per_cu[1])));
-    NMX_TRY((launch_class<2, true>(ctx, a, cfg.teams_per_cu[1])));
+    {
+      StreamSwap on(ctx, split ? ctx->aux : ctx->stream);
+      NMX_TRY((launch_class<2, true>(ctx, a, cfg.teams_per_cu[1])));
+    }
     NMX_TRY((launch_class<1, false>(ctx, a, cfg.teams_per_cu[2])));
-    NMX_TRY((launch_class<1, true>(ctx, a, cfg.teams_per_cu[2])));
+    {
+      StreamSwap on(ctx, split ? ctx->aux : ctx->stream);
+      NMX_TRY((launch_class<1, true>(ctx, a, cfg.teams_per_cu[2])));
+    }
+    if (split) {
+      NMX_HIP(ctx, hipEventRecord(ctx->behind_ev, ctx->aux));
+      NMX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->behind_ev, 0));
+    }
   }
   return NMX_OK;
 }
@@ -1261,7 +1504,7 @@ NMX_API int nmx_batched_gesv(nmx_ctx* ctx, int count, const int32_t* n, const in
   if (count == 0) return NMX_OK;
   void* ws = nullptr;
   NMX_TRY(nmx_workspace(ctx, batched_ws(count, a_elems).total(), &ws));
-  return batched_run(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg, ws);
+  return batched_run(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg, ws, true);
 }
 
 NMX_API int nmx_batched_gesv_noalloc(nmx_ctx* ctx, int count, const int32_t* n, const int64_t* offA,
@@ -1274,8 +1517,9 @@ NMX_API int nmx_batched_gesv_noalloc(nmx_ctx* ctx, int count, const int32_t* n, 
   // the context's device copy of the static scratch array: never grown
   if (!ctx->static_scratch || batched_ws(count, a_elems).total() > ctx->static_scratch_bytes)
     return NMX_ERR_UNSUPPORTED;
+  // one stream: creating the second one on first use would allocate
   return batched_run(ctx, count, n, offA, offB, kind, A, a_elems, B, b_elems, X, info, iters, berr, cfg,
-                     ctx->static_scratch);
+                     ctx->static_scratch, false);
 }
 
 // ---- autotuner: resident workgroups per CU, per size class ---------------------
