@@ -606,18 +606,21 @@ __device__ __forceinline__ void panel_chol_unrolled(float* __restrict__ F, int n
 #pragma unroll
     for (int c = 0; c < kNB; c++) dv[c] = buf_load<float>(rf, (own & (c < nb)) ? 4u * (ro + c) : kBufOob);
   }
+  // a guard, not a break: with an early exit the loop is left rolled and
+  // dv indexed at run time (a copy of all 32 registers per inner step)
 #pragma unroll
   for (int j = 0; j < kNB; j++) {
-    if (j >= nb) break;
-    const float d = readlane_f(dv[j], j);
-    if (!(d > 0.f) && info == 0) info = k + j + 1;
-    const float ljj = sqrtf(guard_f(d));
-    if (lane == j) dv[j] = ljj;
-    else if (lane > j) dv[j] = dv[j] / ljj;
+    if (j < nb) {
+      const float d = readlane_f(dv[j], j);
+      if (!(d > 0.f) && info == 0) info = k + j + 1;
+      const float ljj = sqrtf(guard_f(d));
+      if (lane == j) dv[j] = ljj;
+      else if (lane > j) dv[j] = dv[j] / ljj;
 #pragma unroll
-    for (int c = j + 1; c < kNB; c++) {
-      const float lc = readlane_f(dv[j], c);
-      if (lane > j) dv[c] -= dv[j] * lc;
+      for (int c = j + 1; c < kNB; c++) {
+        const float lc = readlane_f(dv[j], c);
+        if (lane > j) dv[c] -= dv[j] * lc;
+      }
     }
   }
   if (lane < nb) {
@@ -1149,12 +1152,17 @@ __device__ __forceinline__ double residual_wave(const double* __restrict__ A, co
   return den > 0.0 ? rmax / den : (rmax == 0.0 ? 0.0 : INFINITY);
 }
 
-// Two wavefronts per SIMD: left to itself the compiler takes 256 VGPRs and
-// 16-28 AGPRs, which leaves room for one, and a lone wavefront waits out
-// every load of its chain of dependent phases.  Held to 256 registers the
-// larger classes spill a few words a lane.
+// Two wavefronts per SIMD where the registers allow it: left to itself the
+// compiler takes 256 VGPRs and 16-28 AGPRs, which leaves room for one, and
+// a lone wavefront waits out every load of its chain of dependent phases.
+// The unrolled LU panel (n > 128) held to 256 registers spills about ten
+// words to scratch memory per column step, which costs more than the second
+// wavefront gains (14.3 ms against 15.2 for the 20,000-system batch), so
+// those two classes keep one.
 template <int Q, bool CHOL>
-__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2)))
+constexpr int kWavesPerSimd = (CHOL || kRolled<Q>) ? 2 : 1;
+template <int Q, bool CHOL>
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd<Q, CHOL>)))
 batched_wave_kernel(BatchArgs args, int cls) {
   using Lds = typename std::conditional<kRolled<Q>, WaveLdsBlk<64 * Q>, WaveLdsTile<64 * Q>>::type;
   __shared__ Lds lds[kWaves];
