@@ -136,19 +136,25 @@ def batched_leg(args, torch) -> dict:
     berr = torch.zeros(len(n), dtype=torch.float64, device="cuda")
     cfg = _lib.BatchCfg()
     lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def solve(c):
+        _lib.check(lib.nmx_batched_gesv(ctx.handle, len(n), P(N), P(OA), P(OB), P(K), P(a), a.numel(), P(b),
+                                        b.numel(), P(X), P(info), P(iters), P(berr), ctypes.byref(c)),
+                   "nmx_batched_gesv")
+
     teams = None
     if not args.no_autotune:
+        # tuned on this batch, before the timed steps (the size classes share
+        # the chip, so a per-class synthetic sweep does not carry over)
         from numax_amd import autotune
-        teams = autotune.tune_batched(samples=1500, store=False)
+        teams = autotune.tune_batched_on(solve)
         for i, v in enumerate(teams):
             cfg.teams_per_cu[i] = v
         log(f"autotuned teams_per_cu {teams}")
-    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
 
     def step():
-        _lib.check(lib.nmx_batched_gesv(ctx.handle, len(n), P(N), P(OA), P(OB), P(K), P(a), a.numel(), P(b),
-                                        b.numel(), P(X), P(info), P(iters), P(berr), ctypes.byref(cfg)),
-                   "nmx_batched_gesv")
+        solve(cfg)
 
     for _ in range(args.warmup):
         step()

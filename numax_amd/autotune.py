@@ -78,6 +78,49 @@ def tune_batched(device: int = 0, samples: int = 2000, store: bool = True) -> li
     return teams
 
 
+def tune_batched_on(run, candidates=(1, 2, 3, 4, 6, 8), reps: int = 2) -> list:
+    """Tunes ``teams_per_cu`` on the caller's own batch: ``run(cfg)`` solves
+    it once with the given ``nmx_batch_cfg`` (and may be called ~50 times).
+
+    The size classes' launches share the chip (the Cholesky classes run on
+    a second stream beside the LU classes), so the best value for one class
+    depends on the others and on the batch's mix; the synthetic per-class
+    sweep of :func:`tune_batched` cannot see that.  This is one pass of
+    coordinate search from the library's defaults: each class in turn takes
+    the candidate with the shortest wall time (best of ``reps`` runs after a
+    warm-up, the device synchronized around each) with the others held."""
+    import time
+
+    import torch
+    lib = _lib.load()
+    cfg = _lib.BatchCfg()
+    lib.nmx_batch_cfg_default(ctypes.byref(cfg))
+
+    def timed(c) -> float:
+        run(c)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run(c)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    cur = timed(cfg)
+    for i in range(3):
+        start = keep = int(cfg.teams_per_cu[i])
+        for v in candidates:
+            if v == start:  # already timed
+                continue
+            cfg.teams_per_cu[i] = v
+            t = timed(cfg)
+            if t < cur:
+                cur, keep = t, v
+        cfg.teams_per_cu[i] = keep
+    return [int(cfg.teams_per_cu[i]) for i in range(3)]
+
+
 def tune_mpir(n: int, device: int = 0, precision: str = "fp16s", store: bool = True) -> int:
     """Sweeps the mixed-precision panel width (128, 256, 512) at order ``n``;
     returns the fastest."""

@@ -237,3 +237,32 @@ def test_batched_problems_sharing_a_matrix():
     info2 = res2.info.cpu().numpy()
     assert info2[0] == 0 and info2[1] == 0 and info2[2] == -1
     np.testing.assert_array_equal(res2.x.cpu().numpy()[:2 * n0], res.x.cpu().numpy()[:2 * n0])
+
+
+def test_batched_tuned_on_the_batch_still_solves():
+    """autotune.tune_batched_on: coordinate search of teams_per_cu on the
+    caller's own batch; every candidate run solves the batch, and the chosen
+    shape gives the same x as the defaults (problems are independent of the
+    launch shape)."""
+    import ctypes
+
+    from numax_amd import _lib, autotune
+    rng = np.random.default_rng(77)
+    sizes = [int(s) for s in rng.integers(8, 257, size=120)]
+    mats, rhss, kinds = _systems(sizes, seed=78)
+    a, b, n, oa, ob = nl.pack_systems(mats, rhss)
+    calls = []
+
+    def run(cfg):
+        calls.append([int(cfg.teams_per_cu[i]) for i in range(3)])
+        nl.batched_solve(a, b, n, kinds, oa, ob, teams_per_cu=calls[-1])
+
+    teams = autotune.tune_batched_on(run, reps=1)
+    assert len(teams) == 3 and all(t in (1, 2, 3, 4, 6, 8) for t in teams)
+    assert len(calls) == 2 * (1 + 3 * 5)
+    cfg = _lib.BatchCfg()
+    _lib.load().nmx_batch_cfg_default(ctypes.byref(cfg))
+    ref, _, _, _ = _run(mats, rhss, kinds, teams_per_cu=[int(cfg.teams_per_cu[i]) for i in range(3)])
+    res, info, iters, berr = _run(mats, rhss, kinds, teams_per_cu=teams)
+    _check(mats, rhss, kinds, res, info, berr)
+    np.testing.assert_array_equal(res.x.cpu().numpy(), ref.x.cpu().numpy())
