@@ -52,7 +52,7 @@ MEASURED_TRAFFIC = {
     ("mp_gemm", 16384): (249.53 + 170.69) * 2 ** 20,  # next-panel and look-ahead parts, averaged
     # plain FP64 path: the 128-tile trailing updates and the 64-tile in-panel updates, per class launch
     ("gemm", 16384): 108.68 * 2 ** 20,
-    ("batched_solve", 20000): 35686.44 * 2 ** 20,  # the eight class launches of one call
+    ("batched_solve", 20000): (34144.7 + 7892.0) * 2 ** 20,  # the eight class launches of one call
     ("c2_lu_gemm", 8192): 28.04 * 2 ** 20,  # per class launch, FP64 LU + 16-RHS solve
     ("batched_eigh", 65536): (405.22 + 408.13 + 103.09 + 104.12) * 2 ** 20,  # both class launches
     ("batched_lstsq", 4096): (669.27 + 0.50) * 2 ** 20,
