@@ -203,7 +203,7 @@ def batched_leg(args, torch) -> dict:
 
 def host_threads() -> tuple[int, int]:
     """(threads used, cores visible): the CPU baselines use every core this
-    process may run on, at most 16 (a one-GPU box's share of the host)."""
+    process may run on, at most 16 (one GPU's share of an eight-GPU node's host cores)."""
     vis = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     return min(vis, 16), vis
 
