@@ -245,7 +245,11 @@ int nmx_mp_factor_debug(nmx_ctx* ctx, int n, const double* A, int lda, int preci
  * in its own FP32 scratch, taken in index order from a pool of a_elems
  * elements, and a problem whose scratch would not fit the pool (possible
  * only when matrices overlap) is skipped with info -1.  Solutions are
- * written at offB, so problems that share b also share x. */
+ * written at offB, so problems that share b also share x.  The Cholesky
+ * problems run on the context's second stream beside the LU problems, forked
+ * from and joined back into the context's stream within the call (under
+ * graph capture the second stream must already exist from an earlier call,
+ * else the call stays on one stream). */
 typedef struct nmx_batch_cfg {
   int max_iter;         /* refinement steps at most (default 10) */
   double tol;           /* target backward error (default 1e-15) */
