@@ -253,52 +253,66 @@ __global__ void __launch_bounds__(256) getf2_nopiv_kernel(float* __restrict__ A,
 }
 
 // ---------------------------------------------------------------------------
-// Trailing-update operands: |max| -> power-of-two scale, then FP16 / BF16
+// Trailing-update operands: diagonal power-of-two scaling, then FP16 / BF16
 // (hi [, lo]) copies.  L21 stays m x k (row-major); U12 (k x m) is stored
 // transposed (m x k) so both MFMA operands read 8 consecutive K values.
+// Each row of L21 and each column of U12 gets its own scale (the product
+// L21 U12 = D_r^-1 (D_r L21)(U12 D_c) D_c^-1 is unscaled per element in
+// the GEMM's epilogue), so a row or column far below the operand's largest
+// entry still fills the 16-bit format's range instead of losing bits to
+// its exponent floor.
 // ---------------------------------------------------------------------------
-// max |X| over a rows x cols block (ld ldx) into *out (as float bits; NaN
-// never raises it): rows strided over the workgroups, columns over the
-// threads (coalesced), one atomic per workgroup.
-constexpr int kAbsmaxBlocks = 256;
-__device__ __forceinline__ void absmax_rows(int bid, int nblk, int rows, int cols, const float* __restrict__ X,
-                                            int ldx, unsigned* __restrict__ out) {
-  __shared__ float part[4];
-  float m = 0.f;
-  for (int r = bid; r < rows; r += nblk) {
-    const float* row = X + (int64_t)r * ldx;
-    for (int c = threadIdx.x; c < cols; c += 256) {
-      const float v = fabsf(row[c]);
-      if (v > m) m = v;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
-    atomicMax(out, __float_as_uint(m));
-  }
-}
-
-// both operands of an update step in one launch: workgroups [0, ga) take X,
-// the rest Y
-__global__ void __launch_bounds__(256) absmax2_kernel(int ga, int rows, int cols, const float* __restrict__ X, int ldx,
-                                                      unsigned* __restrict__ out, int rows2, int cols2,
-                                                      const float* __restrict__ Y, int ldy,
-                                                      unsigned* __restrict__ out2) {
-  if ((int)blockIdx.x < ga)
-    absmax_rows(blockIdx.x, ga, rows, cols, X, ldx, out);
-  else
-    absmax_rows(blockIdx.x - ga, gridDim.x - ga, rows2, cols2, Y, ldy, out2);
-}
-
 __device__ __forceinline__ float pow2_scale(unsigned maxbits) {
   float m = __uint_as_float(maxbits);
   if (!(m > 0.f) || !isfinite(m)) return 1.0f;
   int e;
   frexpf(m, &e);          // m = f * 2^e, f in [0.5, 1)
   return ldexpf(1.0f, 14 - e);  // max * scale in [2^13, 2^14)
+}
+
+// Scales of one update step: workgroups [0, ga) take L21's rows (one
+// wavefront a row), the rest U12's columns (64 columns a workgroup, its four
+// wavefronts splitting the k rows, combined in LDS).  rs[r] and cs[c] are
+// pow2_scale of the row's / column's largest magnitude (NaN never raises it);
+// rs[ld + r] and cs[ld + c] hold the reciprocals (exact: powers of two).
+__global__ void __launch_bounds__(256) scales2_kernel(int ga, int m, int k, const float* __restrict__ L21, int ldl,
+                                                      float* __restrict__ rs, const float* __restrict__ U12, int ldu,
+                                                      float* __restrict__ cs, int ld) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if ((int)blockIdx.x < ga) {
+    for (int r = blockIdx.x * 4 + w; r < m; r += ga * 4) {
+      const float* row = L21 + (int64_t)r * ldl;
+      float mx = 0.f;
+      for (int c = lane; c < k; c += 64) {
+        const float v = fabsf(row[c]);
+        if (v > mx) mx = v;
+      }
+      for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+      if (lane == 0) {
+        const float sc = pow2_scale(__float_as_uint(mx));
+        rs[r] = sc;
+        rs[ld + r] = 1.0f / sc;
+      }
+    }
+  } else {
+    __shared__ float part[4][64];
+    const int c = (blockIdx.x - ga) * 64 + lane;
+    float mx = 0.f;
+    if (c < m) {
+      for (int r = w; r < k; r += 4) {
+        const float v = fabsf(U12[(int64_t)r * ldu + c]);
+        if (v > mx) mx = v;
+      }
+    }
+    part[w][lane] = mx;
+    __syncthreads();
+    if (w == 0 && c < m) {
+      mx = fmaxf(fmaxf(part[0][lane], part[1][lane]), fmaxf(part[2][lane], part[3][lane]));
+      const float sc = pow2_scale(__float_as_uint(mx));
+      cs[c] = sc;
+      cs[ld + c] = 1.0f / sc;
+    }
+  }
 }
 
 template <int BF>
@@ -316,17 +330,16 @@ struct Lo<1> {
   static __device__ __forceinline__ float back(t x) { return (float)x; }
 };
 
-// dst[r][c] = scaled src[r][c] (transpose = 0) or scaled src[c][r] (transpose = 1);
+// dst[r][c] = sc[r] src[r][c] (transpose = 0) or sc[r] src[c][r] (transpose = 1);
 // dst is rows x cols row-major with ld = cols.
 template <int BF, int SPLIT>
 __device__ __forceinline__ void convert_part(int bid, int nblk, int rows, int cols, const float* __restrict__ src,
-                                             int lds, int transpose, const unsigned* __restrict__ maxbits,
+                                             int lds, int transpose, const float* __restrict__ sc,
                                              typename Lo<BF>::t* __restrict__ hi, typename Lo<BF>::t* __restrict__ lo) {
-  const float s = pow2_scale(*maxbits);
   for (int64_t idx = (int64_t)bid * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
        idx += (int64_t)nblk * blockDim.x) {
     int r = (int)(idx / cols), c = (int)(idx % cols);
-    float v = (transpose ? src[(int64_t)c * lds + r] : src[(int64_t)r * lds + c]) * s;
+    float v = (transpose ? src[(int64_t)c * lds + r] : src[(int64_t)r * lds + c]) * sc[r];
     typename Lo<BF>::t h = Lo<BF>::cvt(v);
     hi[idx] = h;
     if (SPLIT) lo[idx] = Lo<BF>::cvt((v - Lo<BF>::back(h)) * 2048.0f);
@@ -336,18 +349,18 @@ __device__ __forceinline__ void convert_part(int bid, int nblk, int rows, int co
 // A (rows x cols, as is) on workgroups [0, ga), B (transposed) on the rest
 template <int BF, int SPLIT>
 __global__ void convert2_kernel(int ga, int rows, int cols, const float* __restrict__ A, int lda,
-                                const unsigned* __restrict__ maxA, typename Lo<BF>::t* __restrict__ ahi,
+                                const float* __restrict__ sA, typename Lo<BF>::t* __restrict__ ahi,
                                 typename Lo<BF>::t* __restrict__ alo, const float* __restrict__ B, int ldb,
-                                const unsigned* __restrict__ maxB, typename Lo<BF>::t* __restrict__ bhi,
+                                const float* __restrict__ sB, typename Lo<BF>::t* __restrict__ bhi,
                                 typename Lo<BF>::t* __restrict__ blo) {
   if ((int)blockIdx.x < ga)
-    convert_part<BF, SPLIT>(blockIdx.x, ga, rows, cols, A, lda, 0, maxA, ahi, alo);
+    convert_part<BF, SPLIT>(blockIdx.x, ga, rows, cols, A, lda, 0, sA, ahi, alo);
   else
-    convert_part<BF, SPLIT>(blockIdx.x - ga, gridDim.x - ga, rows, cols, B, ldb, 1, maxB, bhi, blo);
+    convert_part<BF, SPLIT>(blockIdx.x - ga, gridDim.x - ga, rows, cols, B, ldb, 1, sB, bhi, blo);
 }
 
 // ---------------------------------------------------------------------------
-// C (FP32, in place) -= (A . B^T) / (sA sB) with A m x k, B n x k in FP16 /
+// C (FP32, in place) -= (A . B^T) iA[row] iB[col] (the reciprocal scales) with A m x k, B n x k in FP16 /
 // BF16, FP32 accumulation on v_mfma_f32_32x32x16_{f16,bf16}.  128x128 C tile
 // per 512-thread workgroup, 64x32 per wavefront (2x1 MFMA tiles, so four
 // wavefronts fit a SIMD's registers and hide the slab loads); the C tile
@@ -377,8 +390,8 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
                                                       const typename Lo<BF>::t* __restrict__ Al,
                                                       const typename Lo<BF>::t* __restrict__ Bh,
                                                       const typename Lo<BF>::t* __restrict__ Bl,
-                                                      const unsigned* __restrict__ maxA,
-                                                      const unsigned* __restrict__ maxB, float* __restrict__ C, int ldc,
+                                                      const float* __restrict__ iA,
+                                                      const float* __restrict__ iB, float* __restrict__ C, int ldc,
                                                       int tilesM, int tilesN) {
   typedef typename Lo<BF>::t lt;
   typedef lt v8 __attribute__((ext_vector_type(8)));
@@ -503,7 +516,9 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
         }
   }
 
-  const float inv = 1.0f / (pow2_scale(*maxA) * pow2_scale(*maxB));
+  // the scales are powers of two: multiplying by their reciprocals is exact
+  const int colj = n0 + wn * 32 + (lane & 31);
+  const float ibc = colj < N ? iB[colj] : 1.0f;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -514,7 +529,7 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
         const int col = n0 + wn * 32 + j * 32 + (lane & 31);
         float prod = acc[i][j][e];
         if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
-        const float out = cpre[i][j][e] - prod * inv;
+        const float out = cpre[i][j][e] - prod * (row < M ? iA[row] : 1.0f) * ibc;
         if ((row < M) & (col < N)) C[(int64_t)row * ldc + col] = out;
       }
 }
@@ -1060,11 +1075,6 @@ __global__ void zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
-__global__ void zero_u32k(unsigned* p, int n) {
-  int i = threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-
 __global__ void cast_f64_f32(int n, const double* __restrict__ A, int lda, float* __restrict__ F, int ldf) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
@@ -1149,7 +1159,8 @@ struct MpFactor {
   float* invU = nullptr;
   int32_t* info = nullptr;
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
-  unsigned* maxbits = nullptr;       // [2] per step
+  float* rs = nullptr;               // per-row scales of L21 and their reciprocals (2n), per step
+  float* cs = nullptr;               // per-column scales of U12 and their reciprocals (2n), per step
   uint64_t* xpub = nullptr;          // TRSV published solution words (n)
   IrDev* ird = nullptr;              // device-side refinement state and TRSV epoch base
   uint32_t eoff = 0;                 // TRSV epoch offset of the next launch in this solve
@@ -1174,7 +1185,8 @@ struct MpBuffers {
   int32_t* ipiv = nullptr;
   int32_t* info = nullptr;
   void* lo = nullptr;
-  unsigned* maxbits = nullptr;
+  float* sc = nullptr;
+  size_t sc_elems = 0;
   uint64_t* xpub = nullptr;
   IrDev* ird = nullptr;
   size_t ird_elems = 0;
@@ -1194,7 +1206,7 @@ static void mp_buffers_free(void* p) {
   hipFree(b->ipiv);
   hipFree(b->info);
   hipFree(b->lo);
-  hipFree(b->maxbits);
+  hipFree(b->sc);
   hipFree(b->xpub);
   hipFree(b->ird);
   hipFree(b->mpub);
@@ -1236,14 +1248,14 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   if (!ctx->capturing) NMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const int nblk = ceil_div(n, TB);
   size_t lo_bytes = (size_t)n * block * 4 * 2;  // A_hi, A_lo, B_hi, B_lo (lo unused unless split)
-  size_t unused = 0, unused2 = 0;
+  size_t unused = 0;
   NMX_TRY(grow(ctx, b->F, b->F_elems, (size_t)n * n, sizeof(float)));
   NMX_TRY(grow(ctx, b->d, b->d_elems, 4 * (size_t)n + 16, sizeof(float)));
   NMX_TRY(grow(ctx, b->inv, b->inv_elems, (size_t)nblk * TB * TB * 2, sizeof(float)));
   NMX_TRY(grow(ctx, b->ipiv, b->piv_elems, 2 * (size_t)n, sizeof(int32_t)));
   NMX_TRY(grow(ctx, b->lo, b->lo_bytes, lo_bytes, 1));
   if (!b->info) NMX_TRY(grow(ctx, b->info, unused, 64, sizeof(int32_t)));
-  if (!b->maxbits) NMX_TRY(grow(ctx, b->maxbits, unused2, 64, sizeof(unsigned)));
+  NMX_TRY(grow(ctx, b->sc, b->sc_elems, 4 * (size_t)n, sizeof(float)));
   if ((size_t)n > b->pub_elems) {
     NMX_TRY(grow(ctx, b->xpub, b->pub_elems, (size_t)n, sizeof(uint64_t)));
     hipMemsetAsync(b->xpub, 0, sizeof(uint64_t) * (size_t)n, ctx->stream);
@@ -1265,7 +1277,8 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.perm = b->ipiv + n;
   f.info = b->info;
   f.lo_buf = b->lo;
-  f.maxbits = b->maxbits;
+  f.rs = b->sc;                  // [0, n) scales, [n, 2n) reciprocals
+  f.cs = b->sc + 2 * (size_t)n;
   f.xpub = b->xpub;
   if (f.pivot == 2) {
     const size_t want = tslu_scratch_bytes(n);
@@ -1331,13 +1344,12 @@ static int mp_convert(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   const float* U12 = f.F + (int64_t)k * n + k + nb;    // nb x m, ld n
   lt* base = (lt*)f.lo_buf;
   const size_t op = (size_t)m * nb;
-  zero_u32k<<<1, 64, 0, ctx->stream>>>(f.maxbits, 2);
   const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
-  // both scale words in one launch, then both conversions in one
-  const int ga = std::min(m, kAbsmaxBlocks), gb = std::min(nb, kAbsmaxBlocks);
-  absmax2_kernel<<<ga + gb, 256, 0, ctx->stream>>>(ga, m, nb, L21, n, f.maxbits, nb, m, U12, n, f.maxbits + 1);
-  convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.maxbits, base, base + op, U12, n,
-                                                               f.maxbits + 1, base + 2 * op, base + 3 * op);
+  // both operands' scales in one launch, then both conversions in one
+  const int ga = std::min(ceil_div(m, 4), 1024), gb = ceil_div(m, 64);
+  scales2_kernel<<<ga + gb, 256, 0, ctx->stream>>>(ga, m, nb, L21, n, f.rs, U12, n, f.cs, n);
+  convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.rs, base, base + op, U12, n,
+                                                               f.cs, base + 2 * op, base + 3 * op);
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
@@ -1355,8 +1367,8 @@ static int mp_gemm_cols(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m, int j0,
   ProfScope ps(ctx, K_MP_GEMM, 2.0 * m * (double)cols * nb * (SPLIT ? 3 : 1),
                8.0 * (double)m * cols + (double)sizeof(lt) * (double)(m + cols) * nb * (SPLIT ? 2 : 1));
   mp_gemm_kernel<BF, SPLIT><<<grid, kMpThreads, 0, ctx->stream>>>(
-      m, cols, nb, base, base + op, base + 2 * op + (size_t)j0 * nb, base + 3 * op + (size_t)j0 * nb, f.maxbits,
-      f.maxbits + 1, f.F + (int64_t)(k + nb) * n + k + nb + j0, n, tm, tn);
+      m, cols, nb, base, base + op, base + 2 * op + (size_t)j0 * nb, base + 3 * op + (size_t)j0 * nb, f.rs + n,
+      f.cs + n + j0, f.F + (int64_t)(k + nb) * n + k + nb + j0, n, tm, tn);
   NMX_HIP(ctx, hipGetLastError());
   return NMX_OK;
 }
@@ -1373,8 +1385,8 @@ static int mp_update(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
 // context's second stream, while the main stream factors panel k+1, a
 // latency-bound run of single-wavefront tournaments that leaves most of the
 // chip to the update.  Panel k+1's interchanges reach the columns right of
-// it only once that update has finished (the conversion buffers and scale
-// words are reused only after that wait too).
+// it only once that update has finished (the conversion buffers and the
+// per-row / per-column scales are reused only after that wait too).
 template <int BF, int SPLIT>
 static int mp_getrf_lookahead(nmx_ctx* ctx, MpFactor& f) {
   const int n = f.n, bw = f.block;
