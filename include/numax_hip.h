@@ -159,8 +159,8 @@ int nmx_gemm_f32(nmx_ctx* ctx, int transa, int transb, int m, int n, int k, floa
  * it serves `solve(a, b)` (numax/linalg/basic.mojo:55-146) when a caller asks
  * for FP64 accuracy at low-precision factorization cost.  Blocked LU of A
  * in FP32 (P A = L U; pivoting per cfg.pivot) with the trailing updates on
- * FP16 / BF16 MFMA (FP32 accumulate, a power-of-two scale per row of L21
- * and per column of U12); then
+ * FP16 / BF16 MFMA (FP32 accumulate, a power-of-two scale per 32-row block
+ * of L21 and per column of U12); then
  * r = b - A x in FP64, x += U^-1 L^-1 P r (or GMRES on that correction
  * equation) until the normwise backward error
  * ||r||_inf / (||A||_inf ||x||_inf) <= tol.

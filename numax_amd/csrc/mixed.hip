@@ -256,11 +256,12 @@ __global__ void __launch_bounds__(256) getf2_nopiv_kernel(float* __restrict__ A,
 // Trailing-update operands: diagonal power-of-two scaling, then FP16 / BF16
 // (hi [, lo]) copies.  L21 stays m x k (row-major); U12 (k x m) is stored
 // transposed (m x k) so both MFMA operands read 8 consecutive K values.
-// Each row of L21 and each column of U12 gets its own scale (the product
-// L21 U12 = D_r^-1 (D_r L21)(U12 D_c) D_c^-1 is unscaled per element in
-// the GEMM's epilogue), so a row or column far below the operand's largest
-// entry still fills the 16-bit format's range instead of losing bits to
-// its exponent floor.
+// Each block of 32 rows of L21 and each column of U12 gets its own scale
+// (the product L21 U12 = D_r^-1 (D_r L21)(U12 D_c) D_c^-1 is unscaled in the
+// GEMM's epilogue, where a lane's 16 outputs of an MFMA tile share one row
+// block and one column), so a block or column far below the operand's
+// largest entry still fills the 16-bit format's range instead of losing bits
+// to its exponent floor.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float pow2_scale(unsigned maxbits) {
   float m = __uint_as_float(maxbits);
@@ -270,29 +271,34 @@ __device__ __forceinline__ float pow2_scale(unsigned maxbits) {
   return ldexpf(1.0f, 14 - e);  // max * scale in [2^13, 2^14)
 }
 
-// Scales of one update step: workgroups [0, ga) take L21's rows (one
-// wavefront a row), the rest U12's columns (64 columns a workgroup, its four
-// wavefronts splitting the k rows, combined in LDS).  rs[r] and cs[c] are
-// pow2_scale of the row's / column's largest magnitude (NaN never raises it);
-// rs[ld + r] and cs[ld + c] hold the reciprocals (exact: powers of two).
+// Scales of one update step: workgroups [0, ga) take L21's 32-row blocks
+// (one workgroup a block, a row per thread group of 8), the rest U12's
+// columns (64 columns a workgroup, its four wavefronts splitting the k rows,
+// combined in LDS).  rs[b] and cs[c] are pow2_scale of the block's /
+// column's largest magnitude (NaN never raises it); rs[ld + b] and cs[ld + c]
+// hold the reciprocals (exact: powers of two).
 __global__ void __launch_bounds__(256) scales2_kernel(int ga, int m, int k, const float* __restrict__ L21, int ldl,
                                                       float* __restrict__ rs, const float* __restrict__ U12, int ldu,
                                                       float* __restrict__ cs, int ld) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if ((int)blockIdx.x < ga) {
-    for (int r = blockIdx.x * 4 + w; r < m; r += ga * 4) {
+    __shared__ float red[4];
+    const int rb = blockIdx.x;  // rows 32 rb .. 32 rb + 31
+    float mx = 0.f;
+    for (int r = 32 * rb + (threadIdx.x >> 3); r < min(m, 32 * rb + 32); r += 32) {
       const float* row = L21 + (int64_t)r * ldl;
-      float mx = 0.f;
-      for (int c = lane; c < k; c += 64) {
+      for (int c = threadIdx.x & 7; c < k; c += 8) {
         const float v = fabsf(row[c]);
         if (v > mx) mx = v;
       }
-      for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-      if (lane == 0) {
-        const float sc = pow2_scale(__float_as_uint(mx));
-        rs[r] = sc;
-        rs[ld + r] = 1.0f / sc;
-      }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lane == 0) red[w] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const float sc = pow2_scale(__float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+      rs[rb] = sc;
+      rs[ld + rb] = 1.0f / sc;
     }
   } else {
     __shared__ float part[4][64];
@@ -330,7 +336,8 @@ struct Lo<1> {
   static __device__ __forceinline__ float back(t x) { return (float)x; }
 };
 
-// dst[r][c] = sc[r] src[r][c] (transpose = 0) or sc[r] src[c][r] (transpose = 1);
+// dst[r][c] = sc[r >> 5] src[r][c] (transpose = 0: a scale per 32-row block)
+// or sc[r] src[c][r] (transpose = 1: a scale per row of dst, a column of src);
 // dst is rows x cols row-major with ld = cols.
 template <int BF, int SPLIT>
 __device__ __forceinline__ void convert_part(int bid, int nblk, int rows, int cols, const float* __restrict__ src,
@@ -339,7 +346,7 @@ __device__ __forceinline__ void convert_part(int bid, int nblk, int rows, int co
   for (int64_t idx = (int64_t)bid * blockDim.x + threadIdx.x; idx < (int64_t)rows * cols;
        idx += (int64_t)nblk * blockDim.x) {
     int r = (int)(idx / cols), c = (int)(idx % cols);
-    float v = (transpose ? src[(int64_t)c * lds + r] : src[(int64_t)r * lds + c]) * sc[r];
+    float v = (transpose ? src[(int64_t)c * lds + r] * sc[r] : src[(int64_t)r * lds + c] * sc[r >> 5]);
     typename Lo<BF>::t h = Lo<BF>::cvt(v);
     hi[idx] = h;
     if (SPLIT) lo[idx] = Lo<BF>::cvt((v - Lo<BF>::back(h)) * 2048.0f);
@@ -360,7 +367,7 @@ __global__ void convert2_kernel(int ga, int rows, int cols, const float* __restr
 }
 
 // ---------------------------------------------------------------------------
-// C (FP32, in place) -= (A . B^T) iA[row] iB[col] (the reciprocal scales) with A m x k, B n x k in FP16 /
+// C (FP32, in place) -= (A . B^T) iA[row / 32] iB[col] (the reciprocal scales) with A m x k, B n x k in FP16 /
 // BF16, FP32 accumulation on v_mfma_f32_32x32x16_{f16,bf16}.  128x128 C tile
 // per 512-thread workgroup, 64x32 per wavefront (2x1 MFMA tiles, so four
 // wavefronts fit a SIMD's registers and hide the slab loads); the C tile
@@ -516,9 +523,16 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
         }
   }
 
-  // the scales are powers of two: multiplying by their reciprocals is exact
+  // the scales are powers of two: multiplying by their reciprocals is exact;
+  // MFMA tile i's 32 rows are one row block of A
   const int colj = n0 + wn * 32 + (lane & 31);
   const float ibc = colj < N ? iB[colj] : 1.0f;
+  float iar[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const int r0 = m0 + wm * 64 + i * 32;
+    iar[i] = r0 < M ? iA[r0 >> 5] : 1.0f;
+  }
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -529,7 +543,7 @@ __global__ void __launch_bounds__(kMpThreads) mp_gemm_kernel(int M, int N, int K
         const int col = n0 + wn * 32 + j * 32 + (lane & 31);
         float prod = acc[i][j][e];
         if (SPLIT) prod += accx[i][j][e] * (1.0f / 2048.0f);
-        const float out = cpre[i][j][e] - prod * (row < M ? iA[row] : 1.0f) * ibc;
+        const float out = cpre[i][j][e] - prod * iar[i] * ibc;
         if ((row < M) & (col < N)) C[(int64_t)row * ldc + col] = out;
       }
 }
@@ -1159,7 +1173,7 @@ struct MpFactor {
   float* invU = nullptr;
   int32_t* info = nullptr;
   void* lo_buf = nullptr;            // FP16/BF16 operand copies
-  float* rs = nullptr;               // per-row scales of L21 and their reciprocals (2n), per step
+  float* rs = nullptr;               // scales of L21's 32-row blocks, then their reciprocals (at + n), per step
   float* cs = nullptr;               // per-column scales of U12 and their reciprocals (2n), per step
   uint64_t* xpub = nullptr;          // TRSV published solution words (n)
   IrDev* ird = nullptr;              // device-side refinement state and TRSV epoch base
@@ -1277,7 +1291,7 @@ static int mp_alloc(nmx_ctx* ctx, MpFactor& f, int n, int block, int prec) {
   f.perm = b->ipiv + n;
   f.info = b->info;
   f.lo_buf = b->lo;
-  f.rs = b->sc;                  // [0, n) scales, [n, 2n) reciprocals
+  f.rs = b->sc;                  // [0, n) scales, [n, 2n) reciprocals (L21: one per 32-row block)
   f.cs = b->sc + 2 * (size_t)n;
   f.xpub = b->xpub;
   if (f.pivot == 2) {
@@ -1346,7 +1360,7 @@ static int mp_convert(nmx_ctx* ctx, MpFactor& f, int k, int nb, int m) {
   const size_t op = (size_t)m * nb;
   const int g1 = (int)std::min<int64_t>(4096, ((int64_t)m * nb + 255) / 256);
   // both operands' scales in one launch, then both conversions in one
-  const int ga = std::min(ceil_div(m, 4), 1024), gb = ceil_div(m, 64);
+  const int ga = ceil_div(m, 32), gb = ceil_div(m, 64);
   scales2_kernel<<<ga + gb, 256, 0, ctx->stream>>>(ga, m, nb, L21, n, f.rs, U12, n, f.cs, n);
   convert2_kernel<BF, SPLIT><<<2 * g1, 256, 0, ctx->stream>>>(g1, m, nb, L21, n, f.rs, base, base + op, U12, n,
                                                                f.cs, base + 2 * op, base + 3 * op);
