@@ -1089,11 +1089,42 @@ __global__ void zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
-__global__ void cast_f64_f32(int n, const double* __restrict__ A, int lda, float* __restrict__ F, int ldf) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n * n) return;
-  const int r = (int)(idx / n), c = (int)(idx % n);
-  F[(int64_t)r * ldf + c] = (float)A[(int64_t)r * lda + c];
+// F = (float) A, a wavefront a row, eight 512-byte loads in flight a lane;
+// with `anorm`, also ||A||_inf (the largest row sum of |a_ij|, each row
+// summed lane-strided then across the wavefront, as the blocked residual
+// does) into *anorm as the bits of a non-negative double.
+constexpr int kCastBlocks = 1024;
+__global__ void __launch_bounds__(256) cast_f64_f32(int n, const double* __restrict__ A, int lda,
+                                                    float* __restrict__ F, int ldf,
+                                                    unsigned long long* __restrict__ anorm) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned long long m = 0;
+  for (int row = blockIdx.x * 4 + wv; row < n; row += gridDim.x * 4) {
+    const double* a = A + (int64_t)row * lda;
+    float* f = F + (int64_t)row * ldf;
+    double s = 0.0;
+    int c = lane;
+    for (; c + 64 * 7 < n; c += 64 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = a[c + 64 * u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        f[c + 64 * u] = (float)v[u];
+        s += fabs(v[u]);
+      }
+    }
+    for (; c < n; c += 64) {
+      const double v = a[c];
+      f[c] = (float)v;
+      s += fabs(v);
+    }
+    if (anorm) {  // uniform
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      m = max(m, (unsigned long long)__double_as_longlong(s));
+    }
+  }
+  if (anorm && lane == 0) atomicMax(anorm, m);
 }
 
 // Device-side state of one classical refinement: its steps run as a fixed
@@ -1485,7 +1516,10 @@ static int tslu_hook(nmx_ctx* ctx, void* user, int n, float* A, int lda, int k, 
   return tslu_panel_launch(ctx, f.ts, f.ts_bytes, n, A, lda, k, nb, ipiv, info, 1);
 }
 
-static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed, int stop_after_transform = 0) {
+// anorm (pivoting paths only): ||A||_inf into *anorm (zeroed by the caller)
+// from the same pass that casts A.
+static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64_t seed, int stop_after_transform = 0,
+                     unsigned long long* anorm = nullptr) {
   const int n = f.n;
   if (f.pivot) {
     // GEPP in FP32 (the cooperative pivoting panel), the block-level trailing
@@ -1493,7 +1527,7 @@ static int mp_factor(nmx_ctx* ctx, MpFactor& f, const double* A, int lda, uint64
     const int64_t total = (int64_t)n * n;
     {
       ProfScope ps(ctx, K_OTHER, 0.0, 12.0 * (double)total);
-      cast_f64_f32<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(n, A, lda, f.F, n);
+      cast_f64_f32<<<std::min(ceil_div(n, 4), kCastBlocks), 256, 0, ctx->stream>>>(n, A, lda, f.F, n, anorm);
       NMX_HIP(ctx, hipGetLastError());
     }
     if (stop_after_transform) return zero_info(ctx, f.info);
@@ -2121,15 +2155,17 @@ NMX_API int nmx_gesv_mpir_nrhs(nmx_ctx* ctx, int n, int nrhs, const double* A, i
   hipEventCreate(&e1);
   hipEventCreate(&e2);
   hipEventRecord(e0, ctx->stream);
-  st = mp_factor(ctx, f, A, lda, cfg.seed);
+  // ||A||_inf: from the pass that casts A when the factor pivots, else its own
+  zero_u64<<<1, 64, 0, ctx->stream>>>(norms + 2 * nrhs, 1);
+  st = mp_factor(ctx, f, A, lda, cfg.seed, 0, f.pivot ? norms + 2 * nrhs : nullptr);
   if (st == NMX_OK) st = mp_couple(ctx, f);
   hipEventRecord(e1, ctx->stream);
   if (st == NMX_OK) st = mp_solve_mat(ctx, f, nrhs, B, ldb, X, ldx, T, Yb, 0);  // X0 = M^-1 B
   double anorm = 0.0;
   if (st == NMX_OK) {
-    zero_u64<<<1, 64, 0, ctx->stream>>>(norms + 2 * nrhs, 1);
-    row_abs_max_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda,
-                                                                                          norms + 2 * nrhs);
+    if (!f.pivot)
+      row_abs_max_kernel<<<std::min(ceil_div(n, 4), kResidualBlocks), 256, 0, ctx->stream>>>(n, A, lda,
+                                                                                            norms + 2 * nrhs);
     unsigned long long ha = 0;
     if (hipMemcpyAsync(&ha, norms + 2 * nrhs, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
