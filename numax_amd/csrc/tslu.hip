@@ -513,12 +513,21 @@ __global__ void __launch_bounds__(kApplyT) tslu_apply_kernel(float* __restrict__
     }
     __syncthreads();  // every row this workgroup moves is read (and L11 is set) before any is written
     if (right) {
-      if (t < ncols) {  // U12 = L11^-1 A12, column t, in pivot order
-        for (int r = 1; r < w; r++) {
-          float acc = a12[r][t];
-          for (int j = 0; j < r; j++) acc = fmaf(-L11[r][j], a12[j][t], acc);
-          a12[r][t] = acc;
+      if (t < ncols) {  // U12 = L11^-1 A12, column t, in pivot order, in registers
+        float y[kTsW];
+#pragma unroll
+        for (int r = 0; r < kTsW; r++) y[r] = a12[r][t];
+#pragma unroll
+        for (int r = 1; r < kTsW; r++) {
+          if (r < w) {  // uniform
+            float acc = y[r];
+#pragma unroll
+            for (int j = 0; j < r; j++) acc = fmaf(-L11[r][j], y[j], acc);
+            y[r] = acc;
+          }
         }
+#pragma unroll
+        for (int r = 1; r < kTsW; r++) a12[r][t] = y[r];
       }
       __syncthreads();
     }
